@@ -1,0 +1,121 @@
+/*
+ * ns_device.h -- thin C-ABI between the C host library and the HIP kernels.
+ *
+ * Every function takes plain pointers and sizes.  Spectral arrays passed
+ * here are interleaved complex FP64 in the reference layout [ny][nx/2+1]
+ * (fftw_complex == double _Complex, ns_spectral_solver.h:5-6); physical
+ * arrays are FP64 [ny][nx].  All functions return 0 on success and a
+ * non-zero SolverStatus value on failure (nsd_error() holds the message);
+ * none of them aborts the process.  One context owns one HIP stream; calls
+ * that return host data synchronise that stream.
+ *
+ * These entry points are the device half of the reference's spectral API
+ * (src/spectral/ns_spectral_solver.h:70-113); the mapping of each public
+ * ns_spectral_* function onto them is in the host library
+ * (navier-stokes-solvers_amd/csrc/host/spectral_api.c).
+ */
+#ifndef NS_DEVICE_H
+#define NS_DEVICE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct NsdContext NsdContext;
+
+enum { NSD_FP64 = 0, NSD_FP32 = 1 };
+enum { NSD_SCHEME_CLASSIC = 0, NSD_SCHEME_IF = 1 };
+
+typedef struct {
+    double Re;
+    double drag;            /* linear drag mu (build-defined extension)            */
+    double cfl_factor;      /* ns_spectral_solver.h:29                              */
+    int scheme;             /* NSD_SCHEME_*                                          */
+    /* hyperviscosity.c:60-102 parameterisation, after update_weights            */
+    int hv_on;
+    int hv_order;
+    double hv_coeff, hv_kc, hv_invbw;
+    int sfd_on;
+    double sfd_strength, sfd_tau;
+    /* forcing.h:12-19 plus the build-defined injection rate                     */
+    int force_on;
+    int force_colored;
+    double force_kmin, force_kmax;
+    double force_energy_target;
+    double force_tau;
+    double force_rate;      /* energy injection rate of omega_hat += sqrt(dt) f   */
+    unsigned long long force_seed;
+} NsdParams;
+
+NsdContext *nsd_create(int nx, int ny, double Lx, double Ly, int precision, char *err,
+                       size_t errlen);
+void nsd_destroy(NsdContext *c);
+const char *nsd_error(const NsdContext *c);
+int nsd_set_params(NsdContext *c, const NsdParams *p);
+int nsd_precision(const NsdContext *c);
+
+/* ---- device-resident state (fast path) */
+int nsd_state_upload(NsdContext *c, const double *omega_hat);
+int nsd_state_download(NsdContext *c, double *omega_hat);
+int nsd_sfd_upload(NsdContext *c, const double *sfd_hat);
+int nsd_sfd_download(NsdContext *c, double *sfd_hat);
+int nsd_state_random_phase(NsdContext *c, unsigned long long seed, double k_peak, double energy);
+int nsd_set_time(NsdContext *c, double t, double dt);
+int nsd_get_time(NsdContext *c, double *t, double *dt, int *status);
+int nsd_set_forcing_counter(NsdContext *c, unsigned long long calls);
+int nsd_get_forcing_counter(NsdContext *c, unsigned long long *calls);
+int nsd_advance(NsdContext *c, int nsteps, int adaptive);
+int nsd_advance_graph(NsdContext *c, int nsteps, int adaptive);
+int nsd_state_derived(NsdContext *c, double *psi_hat, double *u_hat, double *v_hat, double *u,
+                      double *v, double *omega);
+/* out = {E, Z} from the spectral state by Parseval, FP64 accumulation */
+int nsd_state_energy(NsdContext *c, double out[2]);
+int nsd_synchronize(NsdContext *c);
+
+/* timing of the fast path: average duration (ms) of each kernel family over
+ * the last nsd_advance call made with profiling on */
+int nsd_profile(NsdContext *c, int on);
+int nsd_profile_read(NsdContext *c, double *row_ms, double *col_ms, double *step_ms,
+                     int *steps);
+
+/* ---- compat operations on host arrays (reference layouts, FP64) */
+int nsd_grids(NsdContext *c, double *x, double *y, double *kx, double *ky, double *k2, int *mask);
+int nsd_hv_weights(NsdContext *c, int enabled, double coefficient, int order, double cutoff_ratio,
+                   double *weights, double *kmax, double *kc, double *invbw);
+int nsd_physical_to_spectral(NsdContext *c, const double *field, double *field_hat);
+int nsd_spectral_to_physical(NsdContext *c, const double *field_hat, double *field);
+int nsd_dealias(NsdContext *c, double *field_hat);
+int nsd_velocity(NsdContext *c, const double *omega_hat, double *psi_hat, double *u_hat,
+                 double *v_hat);
+int nsd_derivatives(NsdContext *c, const double *omega_hat, double *psi_hat, double *u_hat,
+                    double *v_hat, double *u, double *v, double *omega);
+int nsd_nonlinear(NsdContext *c, const double *omega_hat, const double *psi_hat,
+                  double *nonlinear_hat);
+int nsd_hv_apply(NsdContext *c, double *rhs, const double *state, const double *sfd);
+int nsd_hv_post_step(NsdContext *c, const double *state, double *sfd, double dt);
+int nsd_taylor_green(NsdContext *c, double *u, double *v, double *omega, double *p);
+/* out = {max(u^2+v^2), sum 1/2(u^2+v^2), sum 1/2 omega^2} over R points */
+int nsd_phys_reduce(NsdContext *c, const double *u, const double *v, const double *omega,
+                    double out[3]);
+/* out = {sum|w|^2, sum|w|^2 over sqrt(k2)/max_k > 0.8, max|w|,
+ *        sum|w|^2 over k2 > 0.25 k2[last], count of those modes} */
+int nsd_spec_reduce(NsdContext *c, const double *omega_hat, double out[5]);
+int nsd_spectrum(NsdContext *c, const double *u_hat, const double *v_hat, double *energy_k,
+                 int *count_k, int max_k);
+/* forcing.c:117-184 semantics for one generate call on `calls` previous
+ * calls: active-mode list in reference order, out (H complex) overwritten,
+ * colored (H complex, may be NULL for white noise) updated in place.  It
+ * needs no solver context (NSForcingState carries none), so it runs on the
+ * null stream of the current device. */
+int nsd_forcing_generate(int spectral_size, const int *active_idx, int nactive,
+                         unsigned long long seed_eff, unsigned long long calls, int colored_on,
+                         double alpha, double beta, double energy_target, double *colored,
+                         double *out, char *err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1479 @@
+// nsd_api.hip -- device context, small kernels and the extern "C" layer
+// declared in include/ns_device.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/ns_device.h"
+#include "nsd_kernels.hpp"
+
+using namespace nsd;
+
+namespace {
+
+constexpr int ST_OK = 0, ST_MEM = 3, ST_PARAM = 4;
+
+// ------------------------------------------------------------ small kernels
+
+__global__ void k_fill(double* p, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// reference layout (FP64, full H) -> compact state (kept modes)
+template <typename T>
+__global__ void k_gather_state(Geo G, const double* __restrict__ ref, cx<T>* __restrict__ st) {
+    const size_t n = (size_t)G.nkx_keep * G.nky_keep;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(e / G.nky_keep), cy = (int)(e % G.nky_keep);
+        const int jy = cy <= G.kymax ? cy : cy + (G.ny - G.nky_keep);
+        st[e] = ld_ref<T>(ref, (size_t)jy * G.nkx + kx);
+    }
+}
+
+// compact state -> reference layout (zeros outside the mask)
+template <typename T>
+__global__ void k_scatter_state(Geo G, const cx<T>* __restrict__ st, double* __restrict__ ref) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), kx = (int)(e % G.nkx);
+        cx<T> v = mk(T(0), T(0));
+        if (kept(G, kx, jy)) v = st[(size_t)kx * G.nky_keep + compact_y(G, jy)];
+        st_ref(ref, e, v);
+    }
+}
+
+// spectral_initialization.c:10-49
+__global__ void k_grids(Geo G, double* x, double* y, double* kxa, double* kya, double* k2,
+                        int* mask, unsigned long long* kmax_bits) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    double m = 0.0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        const double kk = kx * kx + ky * ky;
+        if (k2) k2[e] = kk;
+        if (mask) mask[e] = kept(G, i, jy) ? 1 : 0;
+        m = fmax(m, sqrt(kk));
+        if (jy == 0 && kxa) kxa[i] = kx;
+        if (i == 0 && kya) kya[jy] = ky;
+    }
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (x && t < (size_t)G.nx) x[t] = t * G.Lx / G.nx;
+    if (y && t < (size_t)G.ny) y[t] = t * G.Ly / G.ny;
+    __shared__ double red[4];
+    block_max_to(m, kmax_bits, red);
+}
+
+// hyperviscosity.c:98-101
+__global__ void k_hv_weights(Geo G, Phys P, double* w) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        w[e] = hv_weight(P, sqrt(kx * kx + ky * ky));
+    }
+}
+
+// spectral_core.c:8-34 on reference-layout arrays
+__global__ void k_velocity_ref(Geo G, const double* __restrict__ w, double* __restrict__ psi,
+                               double* __restrict__ u, double* __restrict__ v) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        const double k2 = kx * kx + ky * ky;
+        const cx<double> om = ld_ref<double>(w, e);
+        const cx<double> ps = k2 > 1e-12 ? (-1.0 / k2) * om : mk(0.0, 0.0);
+        if (psi) st_ref(psi, e, ps);
+        if (u) st_ref(u, e, mul_i(ps, ky));
+        if (v) st_ref(v, e, mul_i(ps, -kx));
+    }
+}
+
+// spectral_transforms.c:35-43
+__global__ void k_dealias_ref(Geo G, double* f) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        if (!kept(G, i, jy)) {
+            f[2 * e] = 0.0;
+            f[2 * e + 1] = 0.0;
+        }
+    }
+}
+
+// hyperviscosity.c:104-150
+__global__ void k_hv_apply_ref(Geo G, Phys P, double* rhs, const double* st, const double* sfd) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        const double km = sqrt(kx * kx + ky * ky);
+        const cx<double> s = ld_ref<double>(st, e);
+        cx<double> d = mk(0.0, 0.0);
+        if (P.hv_on) d = d + hv_weight(P, km) * s;
+        if (P.sfd_on) {
+            const double sw = sfd_weight(P, km);
+            if (sw > 0.0) d = d + sw * (s - ld_ref<double>(sfd, e));
+        }
+        st_ref(rhs, e, ld_ref<double>(rhs, e) - d);
+    }
+}
+
+// hyperviscosity.c:170-177
+__global__ void k_hv_post_ref(size_t n, double alpha, const double* st, double* sfd) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const cx<double> p = ld_ref<double>(sfd, e);
+        st_ref(sfd, e, (1.0 - alpha) * p + alpha * ld_ref<double>(st, e));
+    }
+}
+
+// spectral_initialization.c:72-95 (physical part)
+__global__ void k_taylor_green(Geo G, double* u, double* v, double* w, double* p) {
+    const size_t n = (size_t)G.nx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nx), i = (int)(e % G.nx);
+        const double x = i * G.Lx / G.nx, y = jy * G.Ly / G.ny;
+        const double sx = sin(2 * kPi * x / G.Lx), cxv = cos(2 * kPi * x / G.Lx);
+        const double sy = sin(2 * kPi * y / G.Ly), cy = cos(2 * kPi * y / G.Ly);
+        u[e] = sx * cy;
+        v[e] = -cxv * sy;
+        w[e] = 2.0 * (2 * kPi / G.Lx) * cxv * cy;
+        p[e] = 0.0;
+    }
+}
+
+// spectral_time_integration.c:14-20 and spectral_analysis.c:8-33
+__global__ void k_phys_reduce(size_t n, const double* u, const double* v, const double* w,
+                              DevScalars* sc) {
+    double m = 0.0, e = 0.0, z = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        if (u && v) {
+            const double s = u[i] * u[i] + v[i] * v[i];
+            m = fmax(m, s);
+            e += 0.5 * s;
+        }
+        if (w) z += 0.5 * w[i] * w[i];
+    }
+    __shared__ double red[4];
+    block_max_to(m, &sc->redmax[0], red);
+    block_sum_to(e, &sc->red[0], red);
+    block_sum_to(z, &sc->red[1], red);
+}
+
+// spectral_analysis.c:36-66 and :69-96 (max_k from k2[last]: quirk Q7)
+__global__ void k_spec_reduce(Geo G, const double* w, DevScalars* sc) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    const double kxl = kx_of(G, G.nkx - 1), kyl = ky_of(G, G.ny - 1);
+    const double k2last = kxl * kxl + kyl * kyl;
+    const double max_k = sqrt(k2last);
+    double tot = 0.0, high = 0.0, mx = 0.0, es = 0.0, cnt = 0.0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        const double k2 = kx * kx + ky * ky;
+        const double a = hypot(w[2 * e], w[2 * e + 1]);
+        const double a2 = a * a;
+        tot += a2;
+        const double kn = max_k > 0.0 ? sqrt(k2) / max_k : 0.0;
+        if (kn > 0.8) high += a2;
+        mx = fmax(mx, a);
+        if (k2 > 0.25 * k2last) {
+            es += a2;
+            cnt += 1.0;
+        }
+    }
+    __shared__ double red[4];
+    block_sum_to(tot, &sc->red[0], red);
+    block_sum_to(high, &sc->red[1], red);
+    block_max_to(mx, &sc->redmax[0], red);
+    block_sum_to(es, &sc->red[2], red);
+    block_sum_to(cnt, &sc->red[3], red);
+}
+
+// spectral_analysis.c:103-123: shell sums of 1/2(|u|^2+|v|^2) over stored modes
+__global__ void k_spectrum(Geo G, const double* u, const double* v, double* ek, int* ck,
+                           int max_k) {
+    const size_t n = (size_t)G.nkx * G.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / G.nkx), i = (int)(e % G.nkx);
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        const int bin = (int)(sqrt(kx * kx + ky * ky) + 0.5);
+        if (bin < max_k) {
+            const double au = hypot(u[2 * e], u[2 * e + 1]), av = hypot(v[2 * e], v[2 * e + 1]);
+            atomicAdd(&ek[bin], 0.5 * (au * au + av * av));
+            atomicAdd(&ck[bin], 1);
+        }
+    }
+}
+
+// Parseval sums over the compact state: E = 1/2 sum_full |w|^2/k^2,
+// Z = 1/2 sum_full |w|^2 (half-plane weights 1 at kx = 0, else 2).
+template <typename T>
+__global__ void k_state_energy(Geo G, const cx<T>* st, DevScalars* sc) {
+    const size_t n = (size_t)G.nkx_keep * G.nky_keep;
+    double e = 0.0, z = 0.0;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int kxi = (int)(q / G.nky_keep), cy = (int)(q % G.nky_keep);
+        const int jy = cy <= G.kymax ? cy : cy + (G.ny - G.nky_keep);
+        const double kx = kx_of(G, kxi), ky = ky_of(G, jy);
+        const double k2 = kx * kx + ky * ky;
+        const double wt = (kxi == 0 || 2 * kxi == G.nx) ? 1.0 : 2.0;
+        const double a2 = (double)st[q].x * st[q].x + (double)st[q].y * st[q].y;
+        z += 0.5 * wt * a2;
+        if (k2 > 1e-12) e += 0.5 * wt * a2 / k2;
+    }
+    __shared__ double red[4];
+    block_sum_to(e, &sc->red[0], red);
+    block_sum_to(z, &sc->red[1], red);
+}
+
+__device__ __forceinline__ unsigned long long splitmix64_at(unsigned long long seed,
+                                                            unsigned long long k) {
+    // k-th output (k >= 1) of the SplitMix64 stream of forcing.c:11-16
+    unsigned long long z = seed + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double uniform_at(unsigned long long seed, unsigned long long k) {
+    return (double)(splitmix64_at(seed, k) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Random-phase initial vorticity (build-defined, same definition as the oracle).
+template <typename T>
+__global__ void k_random_phase(Geo G, unsigned long long seed, double k_peak, cx<T>* st) {
+    const size_t n = (size_t)G.nkx_keep * G.nky_keep;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int kxi = (int)(q / G.nky_keep), cy = (int)(q % G.nky_keep);
+        int jy = cy <= G.kymax ? cy : cy + (G.ny - G.nky_keep);
+        bool conj_of = false;
+        if (kxi == 0 && jy > G.ny / 2) {  // Hermitian column: mirror of (0, ny - jy)
+            jy = G.ny - jy;
+            conj_of = true;
+        }
+        const double kx = kx_of(G, kxi), ky = ky_of(G, jy);
+        const double k2 = kx * kx + ky * ky;
+        cx<T> r = mk(T(0), T(0));
+        if (k2 > 1e-12) {
+            const double km = sqrt(k2), rr = km / k_peak;
+            const double Ek = km * km * km * km * exp(-rr * rr);
+            const double amp = sqrt(km * Ek / kPi);
+            const size_t idx = (size_t)jy * G.nkx + kxi;
+            const double ph = 2.0 * kPi * uniform_at(seed, idx + 1);
+            r = mk(T(amp * cos(ph)), T((conj_of ? -1.0 : 1.0) * amp * sin(ph)));
+        }
+        st[q] = r;
+    }
+}
+
+template <typename T>
+__global__ void k_scale_state(size_t n, cx<T>* st, const DevScalars* sc, double target) {
+    const double e = sc->red[0];
+    const T s = T(e > 0.0 ? sqrt(target / e) : 0.0);
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x)
+        st[q] = s * st[q];
+}
+
+// Forcing draws (forcing.c:117-184 semantics through the counter form of the
+// SplitMix64 stream: on call c with A active modes, mode rank a uses
+// uniforms 2(cA+a)+1 and 2(cA+a)+2), then the build-defined application:
+// Hermitian projection of the kx = 0 column, mask, and scaling to the
+// injection rate.  One workgroup.
+struct ForceArgs {
+    Geo geo;
+    int A;
+    const int* modes;    // reference index j*nkx + i per active mode (reference order)
+    const int* partner;  // rank of the (0, ny-j) partner for kx = 0 modes, else -1
+    unsigned long long seed;
+    int colored;
+    double tau, energy_target, rate;
+    double* ou;          // colored state per active rank (interleaved complex)
+    double* work;        // 2 * A doubles
+    int fbx, fby;
+    int box_T;           // 1: box in FP32, 0: FP64
+    void* fbox;
+    DevScalars* sc;
+};
+
+__global__ void k_force(ForceArgs a) {
+    const Geo& G = a.geo;
+    const unsigned long long c = a.sc->fcount;
+    const double dt = a.sc->dt;
+    double alpha = 0.0, beta = 1.0;
+    if (a.colored) {
+        const double tau = fmax(a.tau, 1e-6);
+        alpha = exp(-dt / tau);
+        beta = sqrt(fmax(1.0 - alpha * alpha, 0.0));
+    }
+    __shared__ double red[16];
+    __shared__ double s_scale;
+    double part = 0.0;
+    for (int r = threadIdx.x; r < a.A; r += blockDim.x) {
+        const unsigned long long k = 2ull * (c * (unsigned long long)a.A + r) + 1ull;
+        double u1 = uniform_at(a.seed, k), u2 = uniform_at(a.seed, k + 1);
+        u1 = fmax(u1, 2.2250738585072014e-308);
+        const double rad = sqrt(-2.0 * log(u1));
+        const double th = 2.0 * kPi * u2;
+        double vr = rad * cos(th), vi = rad * sin(th);
+        if (a.colored) {
+            vr = alpha * a.ou[2 * r] + beta * vr;
+            vi = alpha * a.ou[2 * r + 1] + beta * vi;
+        }
+        a.work[2 * r] = vr;
+        a.work[2 * r + 1] = vi;
+        part += vr * vr + vi * vi;
+    }
+    part = wave_sum(part);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        t = fmax(t, 1e-16);
+        s_scale = sqrt(a.energy_target / t);
+    }
+    __syncthreads();
+    const double sc = s_scale;
+    for (int r = threadIdx.x; r < a.A; r += blockDim.x) {
+        a.work[2 * r] *= sc;
+        a.work[2 * r + 1] *= sc;
+        if (a.colored) {
+            a.ou[2 * r] = a.work[2 * r];
+            a.ou[2 * r + 1] = a.work[2 * r + 1];
+        }
+    }
+    __syncthreads();
+    // Hermitian projection + mask, rate of the projected field
+    double rate = 0.0;
+    double* pr = a.work + 2 * a.A;  // projected values
+    for (int r = threadIdx.x; r < a.A; r += blockDim.x) {
+        const int idx = a.modes[r];
+        const int jy = idx / G.nkx, i = idx % G.nkx;
+        double fr = a.work[2 * r], fi = a.work[2 * r + 1];
+        if (i == 0 || 2 * i == G.nx) {
+            const int p = a.partner[r];
+            if (p >= 0) {
+                fr = 0.5 * (fr + a.work[2 * p]);
+                fi = 0.5 * (fi - a.work[2 * p + 1]);
+            } else if (jy == 0 || 2 * jy == G.ny) {
+                fi = 0.0;
+            }
+        }
+        const double kx = kx_of(G, i), ky = ky_of(G, jy);
+        const double k2 = kx * kx + ky * ky;
+        if (!kept(G, i, jy) || k2 <= 1e-12) fr = fi = 0.0;
+        pr[2 * r] = fr;
+        pr[2 * r + 1] = fi;
+        const double wt = (i == 0 || 2 * i == G.nx) ? 1.0 : 2.0;
+        if (k2 > 1e-12) rate += 0.5 * wt * (fr * fr + fi * fi) / k2;
+    }
+    rate = wave_sum(rate);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rate;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        s_scale = (t > 0.0 && a.rate > 0.0) ? sqrt(a.rate / t) : 0.0;
+    }
+    __syncthreads();
+    const double s2 = s_scale;
+    const int boxn = a.fbx * (2 * a.fby + 1);
+    for (int e = threadIdx.x; e < boxn; e += blockDim.x) {
+        if (a.box_T) reinterpret_cast<float*>(a.fbox)[2 * e] = 0.f, reinterpret_cast<float*>(a.fbox)[2 * e + 1] = 0.f;
+        else reinterpret_cast<double*>(a.fbox)[2 * e] = 0.0, reinterpret_cast<double*>(a.fbox)[2 * e + 1] = 0.0;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < a.A; r += blockDim.x) {
+        const int idx = a.modes[r];
+        const int jy = idx / G.nkx, i = idx % G.nkx;
+        const int b = band_of(G, jy);
+        if (i >= a.fbx || b < -a.fby || b > a.fby) continue;
+        const int e = i * (2 * a.fby + 1) + (b + a.fby);
+        if (a.box_T) {
+            reinterpret_cast<float*>(a.fbox)[2 * e] = float(s2 * pr[2 * r]);
+            reinterpret_cast<float*>(a.fbox)[2 * e + 1] = float(s2 * pr[2 * r + 1]);
+        } else {
+            reinterpret_cast<double*>(a.fbox)[2 * e] = s2 * pr[2 * r];
+            reinterpret_cast<double*>(a.fbox)[2 * e + 1] = s2 * pr[2 * r + 1];
+        }
+    }
+    if (threadIdx.x == 0) a.sc->fcount = c + 1;
+}
+
+// forcing.c:117-184 on a full reference-layout output (compat entry point)
+__global__ void k_force_ref(int A, const int* modes, unsigned long long seed, unsigned long long c,
+                            int colored, double alpha, double beta, double energy_target,
+                            double* col, double* out) {
+    __shared__ double red[16];
+    __shared__ double s_scale;
+    double part = 0.0;
+    for (int r = threadIdx.x; r < A; r += blockDim.x) {
+        const unsigned long long k = 2ull * (c * (unsigned long long)A + r) + 1ull;
+        double u1 = uniform_at(seed, k), u2 = uniform_at(seed, k + 1);
+        u1 = fmax(u1, 2.2250738585072014e-308);
+        const double rad = sqrt(-2.0 * log(u1));
+        const double th = 2.0 * kPi * u2;
+        double vr = rad * cos(th), vi = rad * sin(th);
+        const int idx = modes[r];
+        if (colored) {
+            vr = alpha * col[2 * idx] + beta * vr;
+            vi = alpha * col[2 * idx + 1] + beta * vi;
+            col[2 * idx] = vr;
+            col[2 * idx + 1] = vi;
+        }
+        out[2 * idx] = vr;
+        out[2 * idx + 1] = vi;
+        part += vr * vr + vi * vi;
+    }
+    part = wave_sum(part);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        t = fmax(t, 1e-16);
+        s_scale = sqrt(energy_target / t);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < A; r += blockDim.x) {
+        const int idx = modes[r];
+        out[2 * idx] *= s_scale;
+        out[2 * idx + 1] *= s_scale;
+        if (colored) {
+            col[2 * idx] *= s_scale;
+            col[2 * idx + 1] *= s_scale;
+        }
+    }
+}
+
+// exp(-2 pi i m / n), exact quadrant reduction of the integer ratio
+void twiddle(long m, long n, double* c, double* s) {
+    m %= n;
+    long q = (4 * m) / n, r = 4 * m - q * n;
+    double c0, s0;
+    if (2 * r <= n) {
+        const double a = (kPi / 2.0) * ((double)r / (double)n);
+        c0 = std::cos(a);
+        s0 = std::sin(a);
+    } else {
+        const double a = (kPi / 2.0) * ((double)(n - r) / (double)n);
+        c0 = std::sin(a);
+        s0 = std::cos(a);
+    }
+    double cc, ss;
+    switch (q) {
+    case 0: cc = c0; ss = s0; break;
+    case 1: cc = -s0; ss = c0; break;
+    case 2: cc = -c0; ss = -s0; break;
+    default: cc = s0; ss = -c0; break;
+    }
+    *c = cc;
+    *s = -ss;
+}
+
+unsigned grid_for(size_t n, unsigned bs = 256) {
+    size_t g = (n + bs - 1) / bs;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+
+struct NsdContext {
+    Geo G;
+    int prec;
+    NsdParams P;
+    Phys phys;
+    hipStream_t stream = nullptr;
+    size_t esz;  // bytes per complex element at the working precision
+    void* state = nullptr;
+    void* acc = nullptr;
+    void* stagebuf = nullptr;
+    void* sfd = nullptr;
+    void* mixed[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* nl = nullptr;
+    void* twx = nullptr;
+    void* twy = nullptr;
+    DevScalars* sc = nullptr;
+    double* stH[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* stR[4] = {nullptr, nullptr, nullptr, nullptr};
+    int* fmodes = nullptr;
+    int* fpartner = nullptr;
+    double* fou = nullptr;
+    double* fwork = nullptr;
+    void* fbox = nullptr;
+    int fA = 0, fbx = 0, fby = 0;
+    int prof = 0;
+    std::vector<hipEvent_t> evs;
+    double prof_row = 0, prof_col = 0, prof_step = 0;
+    int prof_steps = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int g_adaptive = -1;
+    unsigned long long gen = 0, g_gen = ~0ull;
+    std::string err;
+};
+
+#define CK(call)                                                                   \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            c->err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+            return ST_MEM;                                                         \
+        }                                                                          \
+    } while (0)
+
+namespace {
+
+size_t H_of(const NsdContext* c) { return (size_t)c->G.nkx * c->G.ny; }
+size_t R_of(const NsdContext* c) { return (size_t)c->G.nx * c->G.ny; }
+size_t C_of(const NsdContext* c) { return (size_t)c->G.nkx_keep * c->G.nky_keep; }
+size_t M_of(const NsdContext* c) { return (size_t)c->G.nkx_keep * c->G.ny; }
+
+int ensure_staging(NsdContext* c) {
+    for (int i = 0; i < 4; ++i) {
+        if (!c->stH[i]) CK(hipMalloc(&c->stH[i], 2 * sizeof(double) * H_of(c)));
+        if (!c->stR[i]) CK(hipMalloc(&c->stR[i], sizeof(double) * R_of(c)));
+    }
+    return ST_OK;
+}
+
+int up(NsdContext* c, void* dst, const void* src, size_t bytes) {
+    if (!src) return ST_OK;
+    CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return ST_OK;
+}
+int down(NsdContext* c, void* dst, const void* src, size_t bytes) {
+    if (!dst) return ST_OK;
+    CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return ST_OK;
+}
+int sync(NsdContext* c) {
+    CK(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+int lastk(NsdContext* c) {
+    CK(hipGetLastError());
+    return ST_OK;
+}
+
+template <typename T>
+ColArgs<T> col_args(NsdContext* c) {
+    ColArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.geo = c->G;
+    a.phys = c->phys;
+    a.fwd_scale = 1.0;
+    a.tw = (const cx<T>*)c->twy;
+    a.nl = (const cx<T>*)c->nl;
+    for (int i = 0; i < 4; ++i) a.mixed[i] = (cx<T>*)c->mixed[i];
+    a.state = (cx<T>*)c->state;
+    a.acc = (cx<T>*)c->acc;
+    a.stagebuf = (cx<T>*)c->stagebuf;
+    a.sfd = (cx<T>*)c->sfd;
+    a.fbox = (const cx<T>*)c->fbox;
+    a.sc = c->sc;
+    return a;
+}
+
+template <typename T>
+RowArgs<T> row_args(NsdContext* c) {
+    RowArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.geo = c->G;
+    a.tw = (const cx<T>*)c->twx;
+    for (int i = 0; i < 4; ++i) a.mixed[i] = (const cx<T>*)c->mixed[i];
+    a.nl = (cx<T>*)c->nl;
+    a.sc = c->sc;
+    return a;
+}
+
+template <typename T>
+void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
+    const int L = c->G.ny;
+    const int NT = L >= 8192 ? 512 : 256;
+    const int lines = NT * 16 / L;
+    const int blocks = (ncols + lines - 1) / lines;
+    const size_t lds = (size_t)lines * (L + L / 16) * sizeof(cx<T>);
+    if (NT == 256)
+        hipLaunchKernelGGL((k_col<T, 256>), dim3(blocks), dim3(256), lds, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_col<T, 512>), dim3(blocks), dim3(512), lds, c->stream, a);
+}
+
+template <typename T>
+void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items) {
+    const int L = c->G.nx;
+    const int NT = L >= 8192 ? 512 : 256;
+    const int lines = NT * 16 / L;
+    const int blocks = (items + lines - 1) / lines;
+    const size_t lds = (size_t)lines * (L + L / 16) * sizeof(cx<T>);
+    if (NT == 256)
+        hipLaunchKernelGGL((k_row<T, 256>), dim3(blocks), dim3(256), lds, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_row<T, 512>), dim3(blocks), dim3(512), lds, c->stream, a);
+}
+
+template <typename T>
+void set_lds_limits(int bytes) {
+    // allow the large dynamic LDS footprints of the long lines (up to the
+    // device's per-workgroup limit); failures surface at launch instead
+    (void)hipFuncSetAttribute((const void*)k_col<T, 256>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_col<T, 512>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_row<T, 256>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_row<T, 512>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipGetLastError();  // attribute calls must not leave a sticky error behind
+}
+
+size_t line_lds_bytes(int L, size_t esz) {
+    const int NT = L >= 8192 ? 512 : 256;
+    return (size_t)(NT * 16 / L) * (L + L / 16) * esz;
+}
+
+// --- typed operations
+
+template <typename T>
+int op_prime(NsdContext* c) {
+    ColArgs<T> a = col_args<T>(c);
+    a.mode = COL_PRIME;
+    launch_col_t<T>(c, a, c->G.nkx_keep);
+    return lastk(c);
+}
+
+template <typename T>
+void enqueue_stage(NsdContext* c, int s, int adaptive) {
+    RowArgs<T> r = row_args<T>(c);
+    r.mode = ROW_STAGE;
+    r.track_umax = (s == 0) ? 1 : 0;
+    ColArgs<T> a = col_args<T>(c);
+    a.mode = COL_STAGE;
+    a.stage = s;
+    a.phys.adaptive = adaptive;
+    const bool prof = c->prof != 0;
+    if (prof) (void)hipEventRecord(c->evs[c->prof_steps * 8 + 2 * s], c->stream);
+    launch_row_t<T>(c, r, c->G.ny / 2);
+    if (prof) (void)hipEventRecord(c->evs[c->prof_steps * 8 + 2 * s + 1], c->stream);
+    launch_col_t<T>(c, a, c->G.nkx_keep);
+}
+
+template <typename T>
+void enqueue_force(NsdContext* c) {
+    if (!c->phys.force_on || c->fA == 0) return;
+    ForceArgs f;
+    memset(&f, 0, sizeof(f));
+    f.geo = c->G;
+    f.A = c->fA;
+    f.modes = c->fmodes;
+    f.partner = c->fpartner;
+    f.seed = c->P.force_seed ? c->P.force_seed : 0x123456789abcdefull;
+    f.colored = c->P.force_colored;
+    f.tau = c->P.force_tau;
+    f.energy_target = c->P.force_energy_target;
+    f.rate = c->P.force_rate;
+    f.ou = c->fou;
+    f.work = c->fwork;
+    f.fbx = c->fbx;
+    f.fby = c->fby;
+    f.box_T = (sizeof(T) == 4) ? 1 : 0;
+    f.fbox = c->fbox;
+    f.sc = c->sc;
+    hipLaunchKernelGGL(k_force, dim3(1), dim3(256), 0, c->stream, f);
+}
+
+// one RK4 step: stage 0 computes dt (adaptive) before the forcing draw,
+// which needs dt for the colored-noise decay and is consumed on stage 3
+template <typename T>
+void enqueue_step(NsdContext* c, int adaptive) {
+    for (int s = 0; s < 4; ++s) {
+        enqueue_stage<T>(c, s, adaptive);
+        if (s == 0) enqueue_force<T>(c);
+    }
+}
+
+template <typename T>
+int op_advance(NsdContext* c, int nsteps, int adaptive) {
+    if (c->prof) {
+        const size_t need = (size_t)nsteps * 8 + 2;
+        while (c->evs.size() < need) {
+            hipEvent_t e;
+            CK(hipEventCreate(&e));
+            c->evs.push_back(e);
+        }
+        c->prof_steps = 0;
+        CK(hipEventRecord(c->evs[(size_t)nsteps * 8], c->stream));
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        enqueue_step<T>(c, adaptive);
+        if (c->prof) c->prof_steps++;
+    }
+    if (c->prof) {
+        CK(hipEventRecord(c->evs[(size_t)nsteps * 8 + 1], c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        double row = 0, col = 0;
+        for (int n = 0; n < nsteps; ++n) {
+            for (int s = 0; s < 4; ++s) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, c->evs[n * 8 + 2 * s], c->evs[n * 8 + 2 * s + 1]);
+                row += ms;
+                hipEvent_t nxt = (s < 3) ? c->evs[n * 8 + 2 * s + 2]
+                                         : (n + 1 < nsteps ? c->evs[(n + 1) * 8]
+                                                           : c->evs[(size_t)nsteps * 8 + 1]);
+                (void)hipEventElapsedTime(&ms, c->evs[n * 8 + 2 * s + 1], nxt);
+                col += ms;
+            }
+        }
+        float tot = 0.f;
+        (void)hipEventElapsedTime(&tot, c->evs[(size_t)nsteps * 8], c->evs[(size_t)nsteps * 8 + 1]);
+        c->prof_row = row / (4.0 * nsteps);
+        c->prof_col = col / (4.0 * nsteps);
+        c->prof_step = tot / nsteps;
+        c->prof_steps = nsteps;
+    }
+    return lastk(c);
+}
+
+template <typename T>
+int op_advance_graph(NsdContext* c, int nsteps, int adaptive) {
+    if (!c->gexec || c->g_adaptive != adaptive || c->g_gen != c->gen) {
+        if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        const int prof = c->prof;
+        c->prof = 0;
+        CK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step<T>(c, adaptive);
+        hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+        c->prof = prof;
+        if (e != hipSuccess) {
+            c->err = std::string("graph capture: ") + hipGetErrorString(e);
+            return ST_MEM;
+        }
+        CK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->g_adaptive = adaptive;
+        c->g_gen = c->gen;
+    }
+    for (int n = 0; n < nsteps; ++n) CK(hipGraphLaunch(c->gexec, c->stream));
+    return ST_OK;
+}
+
+template <typename T>
+int op_upload(NsdContext* c, const double* omega_hat) {
+    if (ensure_staging(c)) return ST_MEM;
+    if (up(c, c->stH[0], omega_hat, 2 * sizeof(double) * H_of(c))) return ST_MEM;
+    hipLaunchKernelGGL(k_gather_state<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], (cx<T>*)c->state);
+    if (op_prime<T>(c)) return ST_MEM;
+    return sync(c);
+}
+
+template <typename T>
+int op_download(NsdContext* c, double* omega_hat, void* src) {
+    if (ensure_staging(c)) return ST_MEM;
+    hipLaunchKernelGGL(k_scatter_state<T>, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream,
+                       c->G, (const cx<T>*)src, c->stH[0]);
+    if (lastk(c) || down(c, omega_hat, c->stH[0], 2 * sizeof(double) * H_of(c))) return ST_MEM;
+    return sync(c);
+}
+
+template <typename T>
+int op_sfd_upload(NsdContext* c, const double* sfd) {
+    if (ensure_staging(c)) return ST_MEM;
+    if (up(c, c->stH[0], sfd, 2 * sizeof(double) * H_of(c))) return ST_MEM;
+    hipLaunchKernelGGL(k_gather_state<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], (cx<T>*)c->sfd);
+    return sync(c);
+}
+
+template <typename T>
+int op_random_phase(NsdContext* c, unsigned long long seed, double k_peak, double energy) {
+    hipLaunchKernelGGL(k_random_phase<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+                       seed, k_peak, (cx<T>*)c->state);
+    CK(hipMemsetAsync(c->sc->red, 0, sizeof(c->sc->red), c->stream));
+    hipLaunchKernelGGL(k_state_energy<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+                       (const cx<T>*)c->state, c->sc);
+    hipLaunchKernelGGL(k_scale_state<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream,
+                       C_of(c), (cx<T>*)c->state, (const DevScalars*)c->sc, energy);
+    if (op_prime<T>(c)) return ST_MEM;
+    return sync(c);
+}
+
+// inverse transforms of up to 4 derived fields of in0 (and in1), from the
+// FP64 reference-layout staging buffers, into FP64 physical outputs
+template <typename T>
+int op_inverse(NsdContext* c, const double* in0, const double* in1, int nout, const int* ops,
+               double* const* outs) {
+    ColArgs<T> a = col_args<T>(c);
+    a.mode = COL_INV;
+    a.in_ref = 1;
+    a.in0_ref = in0;
+    a.in1_ref = in1;
+    a.nout = nout;
+    for (int i = 0; i < nout; ++i) a.ops[i] = ops[i];
+    launch_col_t<T>(c, a, c->G.nkx_keep);
+    RowArgs<T> r = row_args<T>(c);
+    r.mode = ROW_C2R;
+    r.nfields = nout;
+    for (int i = 0; i < nout; ++i) r.out_ref[i] = outs[i];
+    launch_row_t<T>(c, r, c->G.ny);
+    return lastk(c);
+}
+
+template <typename T>
+int op_forward(NsdContext* c, const double* in_phys, double* out_ref, double scale) {
+    RowArgs<T> r = row_args<T>(c);
+    r.mode = ROW_R2C;
+    r.in_ref = in_phys;
+    launch_row_t<T>(c, r, c->G.ny / 2);
+    ColArgs<T> a = col_args<T>(c);
+    a.mode = COL_FWD;
+    a.out_ref = out_ref;
+    a.fwd_scale = scale;
+    launch_col_t<T>(c, a, c->G.nkx);
+    return lastk(c);
+}
+
+template <typename T>
+int op_nonlinear(NsdContext* c, const double* w_ref, const double* psi_ref, double* out_ref) {
+    ColArgs<T> a = col_args<T>(c);
+    a.mode = COL_INV;
+    a.in_ref = 1;
+    a.in0_ref = psi_ref;
+    a.in1_ref = w_ref;
+    a.nout = 4;
+    const int ops[4] = {OP_DX0, OP_DY0, OP_DX1, OP_DY1};  // psi_x, psi_y, omega_x, omega_y
+    for (int i = 0; i < 4; ++i) a.ops[i] = ops[i];
+    launch_col_t<T>(c, a, c->G.nkx_keep);
+    RowArgs<T> r = row_args<T>(c);
+    r.mode = ROW_STAGE;
+    r.track_umax = 0;
+    launch_row_t<T>(c, r, c->G.ny / 2);
+    ColArgs<T> f = col_args<T>(c);
+    f.mode = COL_FWD;
+    f.out_ref = out_ref;
+    f.fwd_scale = -1.0;  // spectral_core.c:108-111
+    launch_col_t<T>(c, f, c->G.nkx);
+    return lastk(c);
+}
+
+template <typename T>
+int op_state_derived(NsdContext* c, double* psi_hat, double* u_hat, double* v_hat, double* u,
+                     double* v, double* omega) {
+    if (ensure_staging(c)) return ST_MEM;
+    hipLaunchKernelGGL(k_scatter_state<T>, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream,
+                       c->G, (const cx<T>*)c->state, c->stH[0]);
+    hipLaunchKernelGGL(k_velocity_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], c->stH[1], c->stH[2], c->stH[3]);
+    const int ops[3] = {OP_U0, OP_V0, OP_ID0};
+    double* outs[3] = {c->stR[0], c->stR[1], c->stR[2]};
+    if (op_inverse<T>(c, c->stH[0], nullptr, 3, ops, outs)) return ST_MEM;
+    // the inverse transforms reused the mixed buffers: restore the invariant
+    // that they hold the derivative fields of the state
+    if (op_prime<T>(c)) return ST_MEM;
+    const size_t hb = 2 * sizeof(double) * H_of(c), rb = sizeof(double) * R_of(c);
+    if (down(c, psi_hat, c->stH[1], hb) || down(c, u_hat, c->stH[2], hb) ||
+        down(c, v_hat, c->stH[3], hb) || down(c, u, c->stR[0], rb) || down(c, v, c->stR[1], rb) ||
+        down(c, omega, c->stR[2], rb))
+        return ST_MEM;
+    return sync(c);
+}
+
+template <typename T>
+int op_state_energy(NsdContext* c, double out[2]) {
+    CK(hipMemsetAsync(c->sc->red, 0, sizeof(c->sc->red), c->stream));
+    hipLaunchKernelGGL(k_state_energy<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+                       (const cx<T>*)c->state, c->sc);
+    double red[8];
+    if (down(c, red, c->sc->red, sizeof(red)) || sync(c)) return ST_MEM;
+    out[0] = red[0];
+    out[1] = red[1];
+    return ST_OK;
+}
+
+// compat calls reuse the mixed buffers; after them the invariant "mixed
+// holds the derivative fields of the state" is restored by a prime pass
+template <typename T>
+int restore_mixed(NsdContext* c) {
+    return op_prime<T>(c);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- extern C
+
+extern "C" {
+
+NsdContext* nsd_create(int nx, int ny, double Lx, double Ly, int precision, char* err,
+                       size_t errlen) {
+    auto fail = [&](const char* m) -> NsdContext* {
+        if (err && errlen) snprintf(err, errlen, "%s", m);
+        return nullptr;
+    };
+    auto pow2 = [](int n) { return n >= 16 && n <= 8192 && (n & (n - 1)) == 0; };
+    if (!pow2(nx) || !pow2(ny)) return fail("nx and ny must be powers of two in [16, 8192]");
+    if (!(Lx > 0.0) || !(Ly > 0.0)) return fail("Lx and Ly must be positive");
+    if (precision != NSD_FP64 && precision != NSD_FP32) return fail("unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
+    NsdContext* c = new NsdContext();
+    Geo& G = c->G;
+    G.nx = nx;
+    G.ny = ny;
+    G.nkx = nx / 2 + 1;
+    G.kxmax = (2 * G.nkx) / 3;  // spectral_initialization.c:46 (quirk Q1)
+    if (G.kxmax > G.nkx - 1) G.kxmax = G.nkx - 1;
+    G.kymax = (2 * ny) / 6;     // spectral_initialization.c:48
+    G.nkx_keep = G.kxmax + 1;
+    G.nky_keep = 2 * G.kymax + 1;
+    G.Lx = Lx;
+    G.Ly = Ly;
+    c->prec = precision;
+    c->esz = precision == NSD_FP32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    memset(&c->P, 0, sizeof(c->P));
+    memset(&c->phys, 0, sizeof(c->phys));
+    c->P.Re = 100.0;
+    c->P.cfl_factor = 0.3;
+    auto bad = [&](hipError_t e) {
+        if (e != hipSuccess) {
+            if (err && errlen) snprintf(err, errlen, "HIP: %s", hipGetErrorString(e));
+            nsd_destroy(c);
+            return true;
+        }
+        return false;
+    };
+    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return nullptr;
+    const size_t C = C_of(c) * c->esz, M = M_of(c) * c->esz;
+    if (bad(hipMalloc(&c->state, C)) || bad(hipMalloc(&c->acc, C)) ||
+        bad(hipMalloc(&c->stagebuf, C)) || bad(hipMalloc(&c->sfd, C)) ||
+        bad(hipMalloc(&c->nl, M)) || bad(hipMalloc(&c->sc, sizeof(DevScalars))))
+        return nullptr;
+    for (int i = 0; i < 4; ++i)
+        if (bad(hipMalloc(&c->mixed[i], M))) return nullptr;
+    if (bad(hipMemset(c->state, 0, C)) || bad(hipMemset(c->sfd, 0, C)) ||
+        bad(hipMemset(c->sc, 0, sizeof(DevScalars))))
+        return nullptr;
+    // twiddle tables
+    for (int d = 0; d < 2; ++d) {
+        const int L = d == 0 ? nx : ny;
+        std::vector<double> tw(2 * (size_t)L);
+        for (int m = 0; m < L; ++m) twiddle(m, L, &tw[2 * m], &tw[2 * m + 1]);
+        void** dst = d == 0 ? &c->twx : &c->twy;
+        if (precision == NSD_FP32) {
+            std::vector<float> tf(tw.begin(), tw.end());
+            if (bad(hipMalloc(dst, tf.size() * sizeof(float))) ||
+                bad(hipMemcpy(*dst, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice)))
+                return nullptr;
+        } else {
+            if (bad(hipMalloc(dst, tw.size() * sizeof(double))) ||
+                bad(hipMemcpy(*dst, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice)))
+                return nullptr;
+        }
+    }
+    int dev = 0, maxlds = 0;
+    if (bad(hipGetDevice(&dev)) ||
+        bad(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)))
+        return nullptr;
+    const size_t need = std::max(line_lds_bytes(nx, c->esz), line_lds_bytes(ny, c->esz));
+    if (need > (size_t)maxlds) {
+        if (err && errlen)
+            snprintf(err, errlen, "transform length needs %zu B of LDS per workgroup, device has %d",
+                     need, maxlds);
+        nsd_destroy(c);
+        return nullptr;
+    }
+    if (precision == NSD_FP32) set_lds_limits<float>(maxlds);
+    else set_lds_limits<double>(maxlds);
+    nsd_set_params(c, &c->P);
+    return c;
+}
+
+void nsd_destroy(NsdContext* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto e : c->evs) (void)hipEventDestroy(e);
+    void* bufs[] = {c->state, c->acc, c->stagebuf, c->sfd, c->nl, c->twx, c->twy, c->sc,
+                    c->mixed[0], c->mixed[1], c->mixed[2], c->mixed[3], c->stH[0], c->stH[1],
+                    c->stH[2], c->stH[3], c->stR[0], c->stR[1], c->stR[2], c->stR[3], c->fmodes,
+                    c->fpartner, c->fou, c->fwork, c->fbox};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* nsd_error(const NsdContext* c) { return c ? c->err.c_str() : "null context"; }
+int nsd_precision(const NsdContext* c) { return c ? c->prec : -1; }
+
+int nsd_set_params(NsdContext* c, const NsdParams* p) {
+    if (!c || !p) return ST_PARAM;
+    if (!(p->Re > 0.0)) {
+        c->err = "Re must be positive";
+        return ST_PARAM;
+    }
+    if (p->scheme == NSD_SCHEME_IF && p->sfd_on) {
+        c->err = "selective frequency damping requires the classic scheme";
+        return ST_PARAM;
+    }
+    c->P = *p;
+    Phys& f = c->phys;
+    f.inv_Re = 1.0 / p->Re;
+    f.Re = p->Re;
+    f.drag = p->drag;
+    f.scheme = p->scheme;
+    f.hv_on = p->hv_on && p->hv_coeff > 0.0;
+    f.hv_order = p->hv_order > 0 ? p->hv_order : 1;
+    f.hv_coeff = p->hv_coeff;
+    f.hv_kc = p->hv_kc;
+    f.hv_invbw = p->hv_invbw;
+    f.sfd_on = p->sfd_on && p->sfd_strength > 0.0;
+    f.sfd_strength = p->sfd_strength;
+    f.sfd_tau = p->sfd_tau;
+    f.cfl = p->cfl_factor;
+    f.dxmin = fmin(c->G.Lx / c->G.nx, c->G.Ly / c->G.ny);
+    f.force_on = 0;
+    c->gen++;
+    // forcing: active modes in reference order (forcing.c:80-93)
+    for (void** b : {(void**)&c->fmodes, (void**)&c->fpartner, (void**)&c->fou,
+                     (void**)&c->fwork, &c->fbox}) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    c->fA = 0;
+    if (p->force_on) {
+        const Geo& G = c->G;
+        const double kmin = fmax(p->force_kmin, 0.0), kmax = fmax(p->force_kmax, kmin);
+        std::vector<int> modes, partner;
+        int fbx = 1, fby = 0;
+        for (int j = 0; j < G.ny; ++j)
+            for (int i = 0; i < G.nkx; ++i) {
+                const double kx = 2.0 * kPi * i / G.Lx;
+                const int b = j <= G.ny / 2 ? j : j - G.ny;
+                const double ky = 2.0 * kPi * b / G.Ly;
+                const double km = std::sqrt(kx * kx + ky * ky);
+                if (km >= kmin && km <= kmax) {
+                    modes.push_back(j * G.nkx + i);
+                    fbx = std::max(fbx, i + 1);
+                    fby = std::max(fby, std::abs(b));
+                }
+            }
+        const int A = (int)modes.size();
+        if (A == 0) {
+            c->err = "forcing shell contains no modes";
+            return ST_PARAM;
+        }
+        partner.assign(A, -1);
+        for (int r = 0; r < A; ++r) {
+            const int j = modes[r] / G.nkx, i = modes[r] % G.nkx;
+            if (i != 0 && 2 * i != G.nx) continue;
+            if (j == 0 || 2 * j == G.ny) continue;
+            const int want = (G.ny - j) * G.nkx + i;
+            for (int q = 0; q < A; ++q)
+                if (modes[q] == want) partner[r] = q;
+        }
+        c->fA = A;
+        c->fbx = fbx;
+        c->fby = fby;
+        const size_t box = (size_t)fbx * (2 * fby + 1);
+        CK(hipMalloc(&c->fmodes, sizeof(int) * A));
+        CK(hipMalloc(&c->fpartner, sizeof(int) * A));
+        CK(hipMalloc(&c->fou, 2 * sizeof(double) * A));
+        CK(hipMalloc(&c->fwork, 4 * sizeof(double) * A));
+        CK(hipMalloc(&c->fbox, c->esz * box));
+        CK(hipMemcpy(c->fmodes, modes.data(), sizeof(int) * A, hipMemcpyHostToDevice));
+        CK(hipMemcpy(c->fpartner, partner.data(), sizeof(int) * A, hipMemcpyHostToDevice));
+        CK(hipMemset(c->fou, 0, 2 * sizeof(double) * A));
+        CK(hipMemset(c->fbox, 0, c->esz * box));
+        f.force_on = 1;
+        f.fbx = fbx;
+        f.fby = fby;
+    }
+    return ST_OK;
+}
+
+#define TYPED(fn, ...) (c->prec == NSD_FP32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+
+int nsd_state_upload(NsdContext* c, const double* w) {
+    if (!c || !w) return ST_PARAM;
+    return TYPED(op_upload, c, w);
+}
+
+int nsd_state_download(NsdContext* c, double* w) {
+    if (!c || !w) return ST_PARAM;
+    return TYPED(op_download, c, w, c->state);
+}
+
+int nsd_sfd_upload(NsdContext* c, const double* s) {
+    if (!c || !s) return ST_PARAM;
+    return TYPED(op_sfd_upload, c, s);
+}
+
+int nsd_sfd_download(NsdContext* c, double* s) {
+    if (!c || !s) return ST_PARAM;
+    return TYPED(op_download, c, s, c->sfd);
+}
+
+int nsd_state_random_phase(NsdContext* c, unsigned long long seed, double k_peak, double energy) {
+    if (!c || !(k_peak > 0.0) || !(energy >= 0.0)) return ST_PARAM;
+    return TYPED(op_random_phase, c, seed, k_peak, energy);
+}
+
+int nsd_set_time(NsdContext* c, double t, double dt) {
+    if (!c) return ST_PARAM;
+    double v[2] = {t, dt};
+    CK(hipMemcpyAsync(c->sc, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+int nsd_get_time(NsdContext* c, double* t, double* dt, int* status) {
+    if (!c) return ST_PARAM;
+    DevScalars s;
+    CK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    if (sync(c)) return ST_MEM;
+    if (t) *t = s.t;
+    if (dt) *dt = s.dt;
+    if (status) *status = s.status;
+    return ST_OK;
+}
+
+int nsd_set_forcing_counter(NsdContext* c, unsigned long long calls) {
+    if (!c) return ST_PARAM;
+    CK(hipMemcpyAsync(&c->sc->fcount, &calls, sizeof(calls), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+int nsd_get_forcing_counter(NsdContext* c, unsigned long long* calls) {
+    if (!c || !calls) return ST_PARAM;
+    CK(hipMemcpyAsync(calls, &c->sc->fcount, sizeof(*calls), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int nsd_advance(NsdContext* c, int nsteps, int adaptive) {
+    if (!c || nsteps < 0) return ST_PARAM;
+    if (nsteps == 0) return ST_OK;
+    return TYPED(op_advance, c, nsteps, adaptive ? 1 : 0);
+}
+
+int nsd_advance_graph(NsdContext* c, int nsteps, int adaptive) {
+    if (!c || nsteps < 0) return ST_PARAM;
+    if (nsteps == 0) return ST_OK;
+    return TYPED(op_advance_graph, c, nsteps, adaptive ? 1 : 0);
+}
+
+int nsd_state_derived(NsdContext* c, double* psi_hat, double* u_hat, double* v_hat, double* u,
+                      double* v, double* omega) {
+    if (!c) return ST_PARAM;
+    return TYPED(op_state_derived, c, psi_hat, u_hat, v_hat, u, v, omega);
+}
+
+int nsd_state_energy(NsdContext* c, double out[2]) {
+    if (!c || !out) return ST_PARAM;
+    return TYPED(op_state_energy, c, out);
+}
+
+int nsd_synchronize(NsdContext* c) {
+    if (!c) return ST_PARAM;
+    return sync(c);
+}
+
+int nsd_profile(NsdContext* c, int on) {
+    if (!c) return ST_PARAM;
+    c->prof = on ? 1 : 0;
+    return ST_OK;
+}
+
+int nsd_profile_read(NsdContext* c, double* row_ms, double* col_ms, double* step_ms, int* steps) {
+    if (!c) return ST_PARAM;
+    if (row_ms) *row_ms = c->prof_row;
+    if (col_ms) *col_ms = c->prof_col;
+    if (step_ms) *step_ms = c->prof_step;
+    if (steps) *steps = c->prof_steps;
+    return ST_OK;
+}
+
+int nsd_grids(NsdContext* c, double* x, double* y, double* kx, double* ky, double* k2, int* mask) {
+    if (!c) return ST_PARAM;
+    if (ensure_staging(c)) return ST_MEM;
+    const Geo& G = c->G;
+    double* dx = c->stR[0];
+    double* dy = dx + G.nx;
+    double* dkx = dy + G.ny;
+    double* dky = dkx + G.nkx;
+    double* dk2 = c->stH[0];
+    int* dmask = (int*)c->stH[1];
+    CK(hipMemsetAsync(&c->sc->redmax[0], 0, sizeof(unsigned long long), c->stream));
+    const unsigned g = std::max(grid_for(H_of(c)), grid_for((size_t)std::max(G.nx, G.ny)));
+    hipLaunchKernelGGL(k_grids, dim3(g), dim3(256), 0, c->stream, G, dx, dy, dkx, dky, dk2, dmask,
+                       &c->sc->redmax[0]);
+    if (lastk(c) || down(c, x, dx, sizeof(double) * G.nx) || down(c, y, dy, sizeof(double) * G.ny) ||
+        down(c, kx, dkx, sizeof(double) * G.nkx) || down(c, ky, dky, sizeof(double) * G.ny) ||
+        down(c, k2, dk2, sizeof(double) * H_of(c)) || down(c, mask, dmask, sizeof(int) * H_of(c)))
+        return ST_MEM;
+    return sync(c);
+}
+
+int nsd_hv_weights(NsdContext* c, int enabled, double coefficient, int order, double cutoff_ratio,
+                   double* weights, double* kmax_out, double* kc_out, double* invbw_out) {
+    if (!c) return ST_PARAM;
+    if (ensure_staging(c)) return ST_MEM;
+    const Geo& G = c->G;
+    CK(hipMemsetAsync(&c->sc->redmax[0], 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_grids, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, G, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, &c->sc->redmax[0]);
+    unsigned long long kb = 0;
+    if (lastk(c) || down(c, &kb, &c->sc->redmax[0], sizeof(kb)) || sync(c)) return ST_MEM;
+    double kmax;
+    memcpy(&kmax, &kb, sizeof(kmax));
+    // hyperviscosity.c:68-79
+    const double ratio = cutoff_ratio < 0.0 ? 0.0 : (cutoff_ratio > 1.0 ? 1.0 : cutoff_ratio);
+    const double kc = ratio * kmax;
+    const double bw = fmax(kmax - kc, 1e-12);
+    const double invbw = bw > 0.0 ? 1.0 / bw : 0.0;
+    if (kmax_out) *kmax_out = kmax;
+    if (kc_out) *kc_out = kc;
+    if (invbw_out) *invbw_out = invbw;
+    if (!weights) return ST_OK;
+    Phys P;
+    memset(&P, 0, sizeof(P));
+    P.hv_on = (enabled && coefficient > 0.0 && kmax > 0.0) ? 1 : 0;
+    P.hv_coeff = coefficient;
+    P.hv_order = order > 0 ? order : 1;
+    P.hv_kc = kc;
+    P.hv_invbw = invbw;
+    hipLaunchKernelGGL(k_hv_weights, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, G, P,
+                       c->stH[0]);
+    if (lastk(c) || down(c, weights, c->stH[0], sizeof(double) * H_of(c))) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_physical_to_spectral(NsdContext* c, const double* field, double* field_hat) {
+    if (!c || !field || !field_hat) return ST_PARAM;
+    if (ensure_staging(c) || up(c, c->stR[0], field, sizeof(double) * R_of(c))) return ST_MEM;
+    if (TYPED(op_forward, c, c->stR[0], c->stH[0], 1.0)) return ST_MEM;
+    if (down(c, field_hat, c->stH[0], 2 * sizeof(double) * H_of(c))) return ST_MEM;
+    if (TYPED(restore_mixed, c)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_spectral_to_physical(NsdContext* c, const double* field_hat, double* field) {
+    if (!c || !field_hat || !field) return ST_PARAM;
+    if (ensure_staging(c) || up(c, c->stH[0], field_hat, 2 * sizeof(double) * H_of(c)))
+        return ST_MEM;
+    const int ops[1] = {OP_ID0};
+    double* outs[1] = {c->stR[0]};
+    if (TYPED(op_inverse, c, c->stH[0], nullptr, 1, ops, outs)) return ST_MEM;
+    if (down(c, field, c->stR[0], sizeof(double) * R_of(c))) return ST_MEM;
+    if (TYPED(restore_mixed, c)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_dealias(NsdContext* c, double* f) {
+    if (!c || !f) return ST_PARAM;
+    if (ensure_staging(c) || up(c, c->stH[0], f, 2 * sizeof(double) * H_of(c))) return ST_MEM;
+    hipLaunchKernelGGL(k_dealias_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0]);
+    if (lastk(c) || down(c, f, c->stH[0], 2 * sizeof(double) * H_of(c))) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_velocity(NsdContext* c, const double* w, double* psi, double* u, double* v) {
+    if (!c || !w) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c);
+    if (ensure_staging(c) || up(c, c->stH[0], w, hb)) return ST_MEM;
+    hipLaunchKernelGGL(k_velocity_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], c->stH[1], c->stH[2], c->stH[3]);
+    if (lastk(c) || down(c, psi, c->stH[1], hb) || down(c, u, c->stH[2], hb) ||
+        down(c, v, c->stH[3], hb))
+        return ST_MEM;
+    return sync(c);
+}
+
+int nsd_derivatives(NsdContext* c, const double* w, double* psi, double* uh, double* vh, double* u,
+                    double* v, double* om) {
+    if (!c || !w) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c), rb = sizeof(double) * R_of(c);
+    if (ensure_staging(c) || up(c, c->stH[0], w, hb)) return ST_MEM;
+    hipLaunchKernelGGL(k_velocity_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], c->stH[1], c->stH[2], c->stH[3]);
+    const int ops[3] = {OP_U0, OP_V0, OP_ID0};  // spectral_core.c:40-42
+    double* outs[3] = {c->stR[0], c->stR[1], c->stR[2]};
+    if (TYPED(op_inverse, c, c->stH[0], nullptr, 3, ops, outs)) return ST_MEM;
+    if (down(c, psi, c->stH[1], hb) || down(c, uh, c->stH[2], hb) || down(c, vh, c->stH[3], hb) ||
+        down(c, u, c->stR[0], rb) || down(c, v, c->stR[1], rb) || down(c, om, c->stR[2], rb))
+        return ST_MEM;
+    if (TYPED(restore_mixed, c)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_nonlinear(NsdContext* c, const double* w, const double* psi, double* out) {
+    if (!c || !w || !psi || !out) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c);
+    if (ensure_staging(c) || up(c, c->stH[0], w, hb) || up(c, c->stH[1], psi, hb)) return ST_MEM;
+    if (TYPED(op_nonlinear, c, c->stH[0], c->stH[1], c->stH[2])) return ST_MEM;
+    if (down(c, out, c->stH[2], hb)) return ST_MEM;
+    if (TYPED(restore_mixed, c)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_hv_apply(NsdContext* c, double* rhs, const double* st, const double* sfd) {
+    if (!c || !rhs || !st) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c);
+    if (ensure_staging(c) || up(c, c->stH[0], rhs, hb) || up(c, c->stH[1], st, hb) ||
+        up(c, c->stH[2], sfd, hb))
+        return ST_MEM;
+    Phys P = c->phys;
+    if (!sfd) P.sfd_on = 0;
+    hipLaunchKernelGGL(k_hv_apply_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G, P,
+                       c->stH[0], c->stH[1], c->stH[2]);
+    if (lastk(c) || down(c, rhs, c->stH[0], hb)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_hv_post_step(NsdContext* c, const double* st, double* sfd, double dt) {
+    if (!c || !st || !sfd) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c);
+    if (!(c->phys.sfd_on && c->phys.sfd_tau > 0.0) || dt <= 0.0) return ST_OK;
+    double alpha = dt / c->phys.sfd_tau;
+    alpha = alpha > 1.0 ? 1.0 : (alpha < 0.0 ? 0.0 : alpha);
+    if (ensure_staging(c) || up(c, c->stH[0], st, hb) || up(c, c->stH[1], sfd, hb)) return ST_MEM;
+    hipLaunchKernelGGL(k_hv_post_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, H_of(c),
+                       alpha, c->stH[0], c->stH[1]);
+    if (lastk(c) || down(c, sfd, c->stH[1], hb)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_taylor_green(NsdContext* c, double* u, double* v, double* w, double* p) {
+    if (!c) return ST_PARAM;
+    const size_t rb = sizeof(double) * R_of(c);
+    if (ensure_staging(c)) return ST_MEM;
+    hipLaunchKernelGGL(k_taylor_green, dim3(grid_for(R_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stR[0], c->stR[1], c->stR[2], c->stR[3]);
+    if (lastk(c) || down(c, u, c->stR[0], rb) || down(c, v, c->stR[1], rb) ||
+        down(c, w, c->stR[2], rb) || down(c, p, c->stR[3], rb))
+        return ST_MEM;
+    return sync(c);
+}
+
+int nsd_phys_reduce(NsdContext* c, const double* u, const double* v, const double* w,
+                    double out[3]) {
+    if (!c || !out) return ST_PARAM;
+    const size_t rb = sizeof(double) * R_of(c);
+    if (ensure_staging(c) || up(c, c->stR[0], u, rb) || up(c, c->stR[1], v, rb) ||
+        up(c, c->stR[2], w, rb))
+        return ST_MEM;
+    CK(hipMemsetAsync(c->sc->red, 0, sizeof(c->sc->red) + sizeof(c->sc->redmax), c->stream));
+    hipLaunchKernelGGL(k_phys_reduce, dim3(grid_for(R_of(c))), dim3(256), 0, c->stream, R_of(c),
+                       u ? c->stR[0] : nullptr, v ? c->stR[1] : nullptr, w ? c->stR[2] : nullptr,
+                       c->sc);
+    DevScalars s;
+    if (lastk(c) || down(c, &s, c->sc, sizeof(s)) || sync(c)) return ST_MEM;
+    memcpy(&out[0], &s.redmax[0], sizeof(double));
+    out[1] = s.red[0];
+    out[2] = s.red[1];
+    return ST_OK;
+}
+
+int nsd_spec_reduce(NsdContext* c, const double* w, double out[5]) {
+    if (!c || !w || !out) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c);
+    if (ensure_staging(c) || up(c, c->stH[0], w, hb)) return ST_MEM;
+    CK(hipMemsetAsync(c->sc->red, 0, sizeof(c->sc->red) + sizeof(c->sc->redmax), c->stream));
+    hipLaunchKernelGGL(k_spec_reduce, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], c->sc);
+    DevScalars s;
+    if (lastk(c) || down(c, &s, c->sc, sizeof(s)) || sync(c)) return ST_MEM;
+    out[0] = s.red[0];
+    out[1] = s.red[1];
+    memcpy(&out[2], &s.redmax[0], sizeof(double));
+    out[3] = s.red[2];
+    out[4] = s.red[3];
+    return ST_OK;
+}
+
+int nsd_spectrum(NsdContext* c, const double* uh, const double* vh, double* ek, int* ck,
+                 int max_k) {
+    if (!c || !uh || !vh || !ek || !ck || max_k < 0) return ST_PARAM;
+    const size_t hb = 2 * sizeof(double) * H_of(c);
+    if (ensure_staging(c) || up(c, c->stH[0], uh, hb) || up(c, c->stH[1], vh, hb)) return ST_MEM;
+    if (max_k == 0) return ST_OK;
+    double* dek = c->stR[0];
+    int* dck = (int*)c->stR[1];
+    if ((size_t)max_k > R_of(c)) return ST_PARAM;
+    CK(hipMemsetAsync(dek, 0, sizeof(double) * max_k, c->stream));
+    CK(hipMemsetAsync(dck, 0, sizeof(int) * max_k, c->stream));
+    hipLaunchKernelGGL(k_spectrum, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       c->stH[0], c->stH[1], dek, dck, max_k);
+    if (lastk(c) || down(c, ek, dek, sizeof(double) * max_k) ||
+        down(c, ck, dck, sizeof(int) * max_k))
+        return ST_MEM;
+    return sync(c);
+}
+
+int nsd_forcing_generate(int H, const int* active_idx, int A, unsigned long long seed,
+                         unsigned long long calls, int colored, double alpha, double beta,
+                         double energy_target, double* col, double* out, char* err,
+                         size_t errlen) {
+    if (H <= 0 || A <= 0 || !active_idx || !out || (colored && !col)) {
+        if (err && errlen) snprintf(err, errlen, "invalid forcing arguments");
+        return ST_PARAM;
+    }
+    int* dm = nullptr;
+    double *dc = nullptr, *dout = nullptr;
+    const size_t hb = 2 * sizeof(double) * (size_t)H;
+    hipError_t e = hipMalloc(&dm, sizeof(int) * A);
+    if (e == hipSuccess) e = hipMalloc(&dout, hb);
+    if (e == hipSuccess && colored) e = hipMalloc(&dc, hb);
+    if (e == hipSuccess) e = hipMemcpy(dm, active_idx, sizeof(int) * A, hipMemcpyHostToDevice);
+    if (e == hipSuccess && colored) e = hipMemcpy(dc, col, hb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(2 * (size_t)H)), dim3(256), 0, 0, dout,
+                           2 * (size_t)H, 0.0);
+        hipLaunchKernelGGL(k_force_ref, dim3(1), dim3(256), 0, 0, A, dm, seed, calls, colored,
+                           alpha, beta, energy_target, dc, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, hb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && colored) e = hipMemcpy(col, dc, hb, hipMemcpyDeviceToHost);
+    if (dm) (void)hipFree(dm);
+    if (dc) (void)hipFree(dc);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) {
+        if (err && errlen) snprintf(err, errlen, "HIP: %s", hipGetErrorString(e));
+        return ST_MEM;
+    }
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+// nsd_fft.hpp -- workgroup-level Stockham FFT building blocks for gfx950.
+//
+// A "line" is one 1-D transform of length L (power of two, 16..8192) held by
+// TPL = L/16 threads, each thread owning 16 complex values.  Between passes
+// the line lives in LDS.  Thread j of a line always reads, and after the
+// last pass always holds, the elements n = j + s*TPL (s = 0..15): the
+// "standard pattern".  Because the first pass reads and the last pass
+// writes that same pattern, pre- and post-processing (spectral derivatives,
+// masks, stage updates, pointwise products) are fused into the registers
+// between transforms without any extra data movement.
+//
+// Pass plan: floor(log16 L) radix-16 passes, then one radix-2/4/8 pass when
+// L is not a power of 16.  Pass with radix R and stride Ns (product of the
+// previous radices): butterfly b reads b + r*L/R, multiplies by
+// W_{Ns R}^{(b mod Ns) r}, does an R-point DFT, and writes (b - b mod Ns)*R
+// + b mod Ns + r*Ns (Stockham autosort, natural-order output).
+// Twiddles come from a table W_L^m = exp(-2 pi i m / L), m < L, computed on
+// the host in FP64 with exact integer angle reduction (FP32 tables are the
+// FP64 values rounded once).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nsd {
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) {
+    cx<T> r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(T s, cx<T> a) { return mk(s * a.x, s * a.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {
+    return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> conjg(cx<T> a) { return mk(a.x, -a.y); }
+// multiply by i*s
+template <typename T>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a, T s) { return mk(-s * a.y, s * a.x); }
+// multiply by DIR*i (DIR = -1: -i, forward; +1: +i, inverse)
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> rot90(cx<T> a) {
+    return DIR < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
+}
+
+// LDS padding: one spare element every 16 keeps the strided Stockham
+// writes of the first passes off a single bank group.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(cx<T>& a, cx<T>& b) {
+    cx<T> t = a - b;
+    a = a + b;
+    b = t;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(cx<T>& a0, cx<T>& a1, cx<T>& a2, cx<T>& a3) {
+    cx<T> s02 = a0 + a2, d02 = a0 - a2;
+    cx<T> s13 = a1 + a3, d13 = rot90<DIR>(a1 - a3);
+    a0 = s02 + s13;
+    a2 = s02 - s13;
+    a1 = d02 + d13;
+    a3 = d02 - d13;
+}
+
+template <typename T>
+struct Consts {
+    static constexpr T c8 = T(0.92387953251128675612818318939678828682241662586);  // cos(pi/8)
+    static constexpr T s8 = T(0.38268343236508977172845998403039886676134456248);  // sin(pi/8)
+    static constexpr T r2 = T(0.70710678118654752440084436210484903928483593768);  // sqrt(1/2)
+};
+
+// multiply by W^e, W = exp(DIR * 2 pi i / 16)
+template <int DIR, int E, typename T>
+__device__ __forceinline__ cx<T> w16(cx<T> a) {
+    constexpr T c8 = Consts<T>::c8, s8 = Consts<T>::s8, r2 = Consts<T>::r2;
+    constexpr T sg = DIR < 0 ? T(-1) : T(1);
+    if constexpr (E == 0) {
+        return a;
+    } else if constexpr (E == 1) {
+        return cmul(a, mk(c8, sg * s8));
+    } else if constexpr (E == 2) {
+        return cmul(a, mk(r2, sg * r2));
+    } else if constexpr (E == 3) {
+        return cmul(a, mk(s8, sg * c8));
+    } else if constexpr (E == 4) {
+        return rot90<DIR>(a);
+    } else if constexpr (E == 6) {
+        return cmul(a, mk(-r2, sg * r2));
+    } else {  // E == 9
+        return cmul(a, mk(-c8, -sg * s8));
+    }
+}
+
+// 8-point DFT (natural order in and out), v0..v7 references.
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cx<T>& v0, cx<T>& v1, cx<T>& v2, cx<T>& v3, cx<T>& v4,
+                                     cx<T>& v5, cx<T>& v6, cx<T>& v7) {
+    // r = 2a + b: 4-point DFTs over a for b = 0,1
+    dft4<DIR>(v0, v2, v4, v6);
+    dft4<DIR>(v1, v3, v5, v7);
+    // twiddle W8^{b k1} for b = 1: k1 = 0..3 -> W16^{0,2,4,6}
+    v3 = w16<DIR, 2>(v3);
+    v5 = w16<DIR, 4>(v5);
+    v7 = w16<DIR, 6>(v7);
+    // 2-point over b; output k = k1 + 4 k2
+    cx<T> o0 = v0 + v1, o4 = v0 - v1;
+    cx<T> o1 = v2 + v3, o5 = v2 - v3;
+    cx<T> o2 = v4 + v5, o6 = v4 - v5;
+    cx<T> o3 = v6 + v7, o7 = v6 - v7;
+    v0 = o0; v1 = o1; v2 = o2; v3 = o3; v4 = o4; v5 = o5; v6 = o6; v7 = o7;
+}
+
+// 16-point DFT on v[base + stride*r], r = 0..15 (natural order in and out).
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(cx<T>* v) {
+    // r = 4a + b, k = k1 + 4 k2
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dft4<DIR>(v[b], v[b + 4], v[b + 8], v[b + 12]);
+    // v[b + 4 k1] now holds y[b][k1]; twiddle by W16^{b k1}
+    v[5] = w16<DIR, 1>(v[5]);
+    v[9] = w16<DIR, 2>(v[9]);
+    v[13] = w16<DIR, 3>(v[13]);
+    v[6] = w16<DIR, 2>(v[6]);
+    v[10] = w16<DIR, 4>(v[10]);
+    v[14] = w16<DIR, 6>(v[14]);
+    v[7] = w16<DIR, 3>(v[7]);
+    v[11] = w16<DIR, 6>(v[11]);
+    v[15] = w16<DIR, 9>(v[15]);
+    // 4-point DFTs over b for each k1: inputs v[4k1 + b], outputs X[k1 + 4 k2]
+    cx<T> o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cx<T> a0 = v[4 * k1 + 0], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
+        dft4<DIR>(a0, a1, a2, a3);
+        o[k1] = a0;
+        o[k1 + 4] = a1;
+        o[k1 + 8] = a2;
+        o[k1 + 12] = a3;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = o[i];
+}
+
+// Line geometry shared by every kernel.
+struct Line {
+    int L;    // transform length
+    int TPL;  // threads per line = L/16
+    int n16;  // number of radix-16 passes
+    int rem;  // trailing radix (1, 2, 4 or 8)
+};
+
+__host__ __device__ inline Line make_line(int L) {
+    Line g;
+    g.L = L;
+    g.TPL = L / 16;
+    int n16 = 0, r = L;
+    while (r >= 16 && (r % 16) == 0) {
+        r /= 16;
+        ++n16;
+    }
+    g.n16 = n16;
+    g.rem = r;
+    return g;
+}
+
+// One Stockham pass of radix R on the thread's 16 registers.
+// If `last`, the outputs stay in v (standard pattern); otherwise they are
+// written to the line's LDS region.
+template <int DIR, int R, typename T>
+__device__ __forceinline__ void stockham_pass(cx<T> (&v)[16], int j, const Line& g, int Ns,
+                                              const cx<T>* __restrict__ tw, cx<T>* lds,
+                                              bool last) {
+    constexpr int Q = 16 / R;
+    const int tstride = g.L / (Ns * R);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = j + q * g.TPL;
+        const int k = b & (Ns - 1);
+        if (Ns > 1) {
+            const int step = k * tstride;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const cx<T> w = tw[r * step];
+                v[q + r * Q] = DIR < 0 ? cmul(v[q + r * Q], w) : cmulc(v[q + r * Q], w);
+            }
+        }
+        if constexpr (R == 16) {
+            dft16<DIR>(v);
+        } else if constexpr (R == 8) {
+            dft8<DIR>(v[q], v[q + Q], v[q + 2 * Q], v[q + 3 * Q], v[q + 4 * Q], v[q + 5 * Q],
+                      v[q + 6 * Q], v[q + 7 * Q]);
+        } else if constexpr (R == 4) {
+            dft4<DIR>(v[q], v[q + Q], v[q + 2 * Q], v[q + 3 * Q]);
+        } else {
+            dft2<DIR>(v[q], v[q + Q]);
+        }
+        if (!last) {
+            const int base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[pad(base + r * Ns)] = v[q + r * Q];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_read_std(cx<T> (&v)[16], const cx<T>* lds, int j, int TPL) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) v[s] = lds[pad(j + s * TPL)];
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_write_std(const cx<T> (&v)[16], cx<T>* lds, int j, int TPL) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) lds[pad(j + s * TPL)] = v[s];
+}
+
+// Full line transform: v (standard pattern, in registers) -> v (standard
+// pattern, natural-order result).  Unnormalised.  DIR = -1 forward
+// (exp(-i...)), +1 inverse.  Every thread of the workgroup must call it
+// (it contains workgroup barriers); the line's LDS region must be free on
+// entry (a barrier is taken first).
+template <int DIR, typename T>
+__device__ __forceinline__ void line_fft(cx<T> (&v)[16], cx<T>* lds, int j, const Line& g,
+                                         const cx<T>* __restrict__ tw) {
+    int Ns = 1;
+    const int passes = g.n16 + (g.rem > 1 ? 1 : 0);
+    for (int p = 0; p < passes; ++p) {
+        const bool last = (p == passes - 1);
+        if (p > 0) {
+            lds_read_std(v, lds, j, g.TPL);
+        }
+        __syncthreads();  // reads of the previous pass (or earlier users) done
+        if (p < g.n16) {
+            stockham_pass<DIR, 16>(v, j, g, Ns, tw, lds, last);
+            Ns *= 16;
+        } else if (g.rem == 8) {
+            stockham_pass<DIR, 8>(v, j, g, Ns, tw, lds, last);
+        } else if (g.rem == 4) {
+            stockham_pass<DIR, 4>(v, j, g, Ns, tw, lds, last);
+        } else {
+            stockham_pass<DIR, 2>(v, j, g, Ns, tw, lds, last);
+        }
+        if (!last) __syncthreads();
+    }
+}
+
+// XCD-aware block order: MI355X dispatches workgroup b to XCD b % 8; this
+// gives each XCD a contiguous range of logical tiles so that neighbouring
+// tiles (which share cache lines of the transposed intermediates) meet in
+// the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    if (nb & 7) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+}  // namespace nsd

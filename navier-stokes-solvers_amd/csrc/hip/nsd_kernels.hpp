@@ -1,0 +1,561 @@
+// nsd_kernels.hpp -- HIP kernels of the pseudospectral vorticity step.
+//
+// Device data layout (N = nx = ny shown; rectangular grids work the same):
+//   state, acc, stage, sfd   compact spectral fields: only the modes kept by
+//                            the 2/3 mask (kx <= kxmax, |ky| <= kymax),
+//                            [kx][cy] with cy = ky for ky <= kymax and
+//                            ky - (ny - nky_keep) for negative ky.
+//   mixed[0..3], nl          "mixed" fields, transformed along y only:
+//                            [kx][y], kx < nkx_keep, y contiguous.
+//   physical                 [y][x] row-major (only for diagnostics/compat).
+// Reference layouts (host mirrors, compat staging) are [ky][kx] with
+// nkx = nx/2+1 columns (ns_spectral_solver.h:22-68).
+//
+// Per RK stage two kernels run:
+//   k_row_stage : for each pair of rows, inverse x-transform of the four
+//                 derivative fields (two complex FFTs per row using the
+//                 two-real-signals-in-one-complex trick), the Jacobian
+//                 J = psi_x omega_y - psi_y omega_x, the CFL speed max, and
+//                 the forward x-transform of J (both rows in one complex FFT).
+//   k_col_stage : for each kept kx column, forward y-transform of J, the
+//                 normalisation/dealias/negation of spectral_core.c:106-111,
+//                 the RK stage update (classic RK4 of
+//                 spectral_time_integration.c:50-100 with hyperviscosity and
+//                 SFD, or the integrating-factor variant), forcing on the
+//                 last stage, then the four derivative spectra of the new
+//                 state and their inverse y-transforms.
+#pragma once
+
+#include "nsd_fft.hpp"
+
+namespace nsd {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Geo {
+    int nx, ny, nkx;
+    int kxmax, kymax, nkx_keep, nky_keep;
+    double Lx, Ly;
+};
+
+struct Phys {
+    double inv_Re;
+    double Re;
+    double drag;
+    int scheme;  // 0 classic RK4 (reference), 1 integrating factor (Lawson RK4)
+    int hv_on;
+    int hv_order;
+    double hv_coeff, hv_kc, hv_invbw;
+    int sfd_on;
+    double sfd_strength, sfd_tau;
+    int adaptive;
+    double cfl, dxmin;
+    int force_on;
+    int fbx, fby;  // forcing box: kx in [0, fbx), ky in [-fby, fby]
+};
+
+struct DevScalars {
+    double t;
+    double dt;
+    unsigned long long umax2_bits;  // max(u^2+v^2) of the stage-1 field, as ordered bits
+    unsigned long long fcount;      // forcing generator call counter
+    double red[8];                  // reduction outputs (sums)
+    unsigned long long redmax[4];   // reduction outputs (max, as ordered bits)
+    int status;                     // 0 ok, 2 diverged (SolverStatus values)
+    int pad_;
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double kx_of(const Geo& g, int i) { return 2.0 * kPi * i / g.Lx; }
+__device__ __forceinline__ int band_of(const Geo& g, int j) { return j <= g.ny / 2 ? j : j - g.ny; }
+__device__ __forceinline__ double ky_of(const Geo& g, int j) {
+    return 2.0 * kPi * band_of(g, j) / g.Ly;
+}
+// spectral_initialization.c:46-49
+__device__ __forceinline__ bool kept(const Geo& g, int i, int j) {
+    const int b = band_of(g, j);
+    return i <= g.kxmax && (b < 0 ? -b : b) <= g.kymax;
+}
+__device__ __forceinline__ int compact_y(const Geo& g, int j) {
+    return j <= g.kymax ? j : j - (g.ny - g.nky_keep);
+}
+
+template <typename T>
+__device__ __forceinline__ T powi(T x, int n) {
+    T r = T(1);
+    for (int i = 0; i < n; ++i) r *= x;
+    return r;
+}
+
+// hyperviscosity.c:90-101
+__device__ __forceinline__ double hv_weight(const Phys& p, double kmag) {
+    if (!p.hv_on || kmag <= p.hv_kc) return 0.0;
+    return p.hv_coeff * powi((kmag - p.hv_kc) * p.hv_invbw, p.hv_order);
+}
+// hyperviscosity.c:124-136
+__device__ __forceinline__ double sfd_weight(const Phys& p, double kmag) {
+    if (!p.sfd_on || p.hv_invbw <= 0.0 || kmag <= p.hv_kc) return 0.0;
+    double s = (kmag - p.hv_kc) * p.hv_invbw;
+    s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+    return p.sfd_strength * powi(s, p.hv_order);
+}
+
+// spectral_time_integration.c:22-28 with MIN_DT/MAX_DT of common_types.h:32-33
+__device__ __forceinline__ double cfl_dt(const Phys& p, double umax) {
+    const double dt_cfl = p.cfl * p.dxmin / (umax + 1e-12);
+    const double dt_visc = 0.5 * p.Re * p.dxmin * p.dxmin;
+    double dt = fmin(dt_cfl, dt_visc);
+    dt = fmax(dt, 1e-8);
+    dt = fmin(dt, 0.1);
+    return dt;
+}
+
+__device__ __forceinline__ unsigned long long ordered_bits(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide max of non-negative doubles folded into *dst with one atomic per
+// workgroup (non-negative doubles order like their bit patterns; NaN and Inf
+// order above every finite value, so divergence is never lost).
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst, double* red_lds) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red_lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red_lds[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = red_lds[i] > m ? red_lds[i] : m;
+        if (m != m) m = __longlong_as_double(0x7ff8000000000000LL);
+        atomicMax(dst, ordered_bits(m));
+    }
+}
+
+__device__ __forceinline__ void block_sum_to(double v, double* dst, double* red_lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red_lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red_lds[i];
+        atomicAdd(dst, s);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> ld(const cx<T>* p, size_t i) {
+    return p[i];
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld_ref(const double* p, size_t i) {  // interleaved FP64
+    return mk(T(p[2 * i]), T(p[2 * i + 1]));
+}
+template <typename T>
+__device__ __forceinline__ void st_ref(double* p, size_t i, cx<T> v) {
+    p[2 * i] = double(v.x);
+    p[2 * i + 1] = double(v.y);
+}
+
+// ------------------------------------------------------------ column side
+
+enum ColMode { COL_STAGE = 0, COL_PRIME = 1, COL_FWD = 2, COL_INV = 3 };
+
+// Field operations applied in spectral space before an inverse y-transform
+// (COL_INV), derived from the reference's spectral_core.c:8-34,62-99.
+enum FieldOp {
+    OP_ID0 = 0,   // in0
+    OP_ID1 = 1,   // in1
+    OP_DX0 = 2,   // i kx in0
+    OP_DY0 = 3,   // i ky in0
+    OP_DX1 = 4,   // i kx in1
+    OP_DY1 = 5,   // i ky in1
+    OP_PSI0 = 6,  // psi_hat = -in0/k^2 (quirk: sign of spectral_core.c:18)
+    OP_U0 = 7,    // u_hat = i ky psi_hat(in0)
+    OP_V0 = 8,    // v_hat = -i kx psi_hat(in0)
+};
+
+template <typename T>
+struct ColArgs {
+    Geo geo;
+    Phys phys;
+    int mode;
+    int stage;       // 0..3 for COL_STAGE
+    int nout;        // number of mixed outputs (COL_INV)
+    int ops[4];      // FieldOp per output (COL_INV)
+    int in_ref;      // COL_INV inputs / COL_FWD output in FP64 reference layout
+    double fwd_scale;  // COL_FWD: factor applied with the 1/(nx ny) normalisation
+    const cx<T>* tw;  // twiddles for length ny
+    const cx<T>* nl;  // mixed input (COL_STAGE, COL_FWD)
+    cx<T>* mixed[4];  // mixed outputs
+    cx<T>* state;     // compact state (read; written on the last stage)
+    cx<T>* acc;       // compact RK accumulator
+    cx<T>* stagebuf;  // compact stage state (classic scheme)
+    cx<T>* sfd;       // compact SFD filter state (classic scheme, if on)
+    const cx<T>* fbox;  // forcing box [fbx][2 fby + 1], already scaled to the injection rate
+    const double* in0_ref;
+    const double* in1_ref;
+    const cx<T>* in0;  // compact inputs for COL_INV (if !in_ref)
+    double* out_ref;   // COL_FWD output, FP64 reference layout, all nkx columns
+    DevScalars* sc;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> apply_op(int op, cx<T> a, cx<T> b, T kx, T ky, T k2) {
+    switch (op) {
+    case OP_ID0: return a;
+    case OP_ID1: return b;
+    case OP_DX0: return mul_i(a, kx);
+    case OP_DY0: return mul_i(a, ky);
+    case OP_DX1: return mul_i(b, kx);
+    case OP_DY1: return mul_i(b, ky);
+    default: break;
+    }
+    const cx<T> psi = k2 > T(1e-12) ? (T(-1) / k2) * a : mk(T(0), T(0));
+    if (op == OP_PSI0) return psi;
+    if (op == OP_U0) return mul_i(psi, ky);
+    return mul_i(psi, -kx);  // OP_V0
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
+    const Geo& G = a.geo;
+    const Line g = make_line(G.ny);
+    const int lines = NT / g.TPL;
+    const int m = threadIdx.x / g.TPL, j = threadIdx.x - m * g.TPL;
+    cx<T>* lds = smem + m * (g.L + g.L / 16);
+    const int kxi = blockIdx.x * lines + m;
+    const int ncols = (a.mode == COL_FWD) ? G.nkx : G.nkx_keep;
+    const bool active = kxi < ncols;
+    const bool has_data = active && kxi < G.nkx_keep;
+    const T kx = T(kx_of(G, kxi));
+    const size_t mbase = (size_t)kxi * G.ny;
+    const cx<T> zero = mk(T(0), T(0));
+
+    cx<T> v[16];
+    cx<T> w[16];
+
+    // ---- forward y-transform of the mixed nonlinear field
+    if (a.mode == COL_STAGE || a.mode == COL_FWD) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) v[s] = has_data ? a.nl[mbase + j + s * g.TPL] : zero;
+        line_fft<-1>(v, lds, j, g, a.tw);
+    }
+
+    if (a.mode == COL_FWD) {
+        // spectral_transforms.c:13-21: x 1/(nx ny), then zero outside the mask
+        const T norm = T(a.fwd_scale / ((double)G.nx * (double)G.ny));
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int jy = j + s * g.TPL;
+            cx<T> r = (has_data && kept(G, kxi, jy)) ? norm * v[s] : zero;
+            if (active) st_ref(a.out_ref, (size_t)jy * G.nkx + kxi, r);
+        }
+        return;
+    }
+
+    // ---- spectral state of this column after the stage (or as is)
+    double dt = 0.0;
+    if (a.mode == COL_STAGE) {
+        if (a.stage == 0 && a.phys.adaptive) {
+            const double u2 = __longlong_as_double((long long)a.sc->umax2_bits);
+            dt = cfl_dt(a.phys, sqrt(u2));
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.sc->dt = dt;
+                if (!(u2 <= 1.0e300)) a.sc->status = 2;  // SOLVER_DIVERGED
+            }
+        } else {
+            dt = a.sc->dt;
+        }
+    }
+    const T h = T(dt);
+    const T norm = T(1.0 / ((double)G.nx * (double)G.ny));
+    const double sfd_alpha = (a.phys.sfd_tau > 0.0) ? fmin(fmax(dt / a.phys.sfd_tau, 0.0), 1.0) : 0.0;
+    const T sqh = T(sqrt(dt));
+
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int jy = j + s * g.TPL;
+        w[s] = zero;
+        if (!(has_data && kept(G, kxi, jy))) continue;
+        const size_t ci = (size_t)kxi * G.nky_keep + compact_y(G, jy);
+        const T ky = T(ky_of(G, jy));
+        if (a.mode == COL_PRIME) {
+            w[s] = a.state[ci];
+            continue;
+        }
+        if (a.mode == COL_INV) {
+            w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : a.in0[ci];
+            continue;
+        }
+        // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
+        const cx<T> N = (-norm) * v[s];
+        const cx<T> w0 = a.state[ci];
+        const double k2d = (double)kx * kx + (double)ky * ky;
+        const double kmag = sqrt(k2d);
+        cx<T> wn;
+        if (a.phys.scheme == 0) {
+            // classic RK4 (spectral_time_integration.c:50-100) with
+            // hyperviscosity_apply (hyperviscosity.c:104-150) and drag
+            const cx<T> ws = (a.stage == 0) ? w0 : a.stagebuf[ci];
+            const T lin = T(-(k2d * a.phys.inv_Re + a.phys.drag + hv_weight(a.phys, kmag)));
+            cx<T> ks = N + lin * ws;
+            const T sw = T(sfd_weight(a.phys, kmag));
+            if (sw > T(0)) ks = ks - sw * (ws - a.sfd[ci]);
+            if (a.stage == 0) {
+                a.acc[ci] = w0 + (h / T(6)) * ks;
+                wn = w0 + (T(0.5) * h) * ks;
+            } else if (a.stage == 1) {
+                a.acc[ci] = a.acc[ci] + (h / T(3)) * ks;
+                wn = w0 + (T(0.5) * h) * ks;
+            } else if (a.stage == 2) {
+                a.acc[ci] = a.acc[ci] + (h / T(3)) * ks;
+                wn = w0 + h * ks;
+            } else {
+                wn = a.acc[ci] + (h / T(6)) * ks;
+                if (a.phys.sfd_on && a.phys.sfd_tau > 0.0) {  // hyperviscosity.c:152-178
+                    const T al = T(sfd_alpha);
+                    a.sfd[ci] = (T(1) - al) * a.sfd[ci] + al * wn;
+                }
+            }
+            if (a.stage < 3) a.stagebuf[ci] = wn;
+        } else {
+            // integrating factor, L = -(k^2/Re + w_hv + mu)
+            const double L = -(k2d * a.phys.inv_Re + a.phys.drag + hv_weight(a.phys, kmag));
+            const T E2 = T(exp(L * 0.5 * dt));
+            const T E = E2 * E2;
+            if (a.stage == 0) {
+                a.acc[ci] = E * w0 + (h / T(6) * E) * N;
+                wn = E2 * (w0 + (T(0.5) * h) * N);
+            } else if (a.stage == 1) {
+                a.acc[ci] = a.acc[ci] + (h / T(3) * E2) * N;
+                wn = E2 * w0 + (T(0.5) * h) * N;
+            } else if (a.stage == 2) {
+                a.acc[ci] = a.acc[ci] + (h / T(3) * E2) * N;
+                wn = E * w0 + (h * E2) * N;
+            } else {
+                wn = a.acc[ci] + (h / T(6)) * N;
+            }
+        }
+        if (a.stage == 3) {
+            if (a.phys.force_on && kxi < a.phys.fbx) {
+                const int b = band_of(G, jy);
+                if (b >= -a.phys.fby && b <= a.phys.fby)
+                    wn = wn + sqh * a.fbox[kxi * (2 * a.phys.fby + 1) + (b + a.phys.fby)];
+            }
+            a.state[ci] = wn;
+        }
+        w[s] = wn;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (a.mode == COL_STAGE && a.stage == 3) {
+            a.sc->t += dt;
+            a.sc->umax2_bits = 0ull;
+        } else if (a.mode == COL_PRIME) {
+            a.sc->umax2_bits = 0ull;
+        }
+    }
+
+    // ---- derivative spectra and inverse y-transforms
+    const int nout = (a.mode == COL_INV) ? a.nout : 4;
+    for (int f = 0; f < nout; ++f) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int jy = j + s * g.TPL;
+            const T ky = T(ky_of(G, jy));
+            const T k2 = kx * kx + ky * ky;
+            if (a.mode == COL_INV) {
+                cx<T> b = zero;
+                if (a.in1_ref && has_data && kept(G, kxi, jy))
+                    b = ld_ref<T>(a.in1_ref, (size_t)jy * G.nkx + kxi);
+                v[s] = apply_op(a.ops[f], w[s], b, kx, ky, k2);
+            } else {
+                // f: 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99)
+                const cx<T> psi = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                const cx<T> src = (f < 2) ? psi : w[s];
+                v[s] = mul_i(src, (f & 1) ? ky : kx);
+            }
+        }
+        line_fft<+1>(v, lds, j, g, a.tw);
+        if (has_data) {
+            cx<T>* out = a.mixed[f];
+#pragma unroll
+            for (int s = 0; s < 16; ++s) out[mbase + j + s * g.TPL] = v[s];
+        }
+    }
+}
+
+// --------------------------------------------------------------- row side
+
+enum RowMode { ROW_STAGE = 0, ROW_C2R = 1, ROW_R2C = 2 };
+
+template <typename T>
+struct RowArgs {
+    Geo geo;
+    int mode;
+    int nfields;           // ROW_C2R: number of mixed fields to transform
+    int track_umax;        // ROW_STAGE: fold max(u^2+v^2) into sc->umax2_bits
+    const cx<T>* tw;       // twiddles for length nx
+    const cx<T>* mixed[4];
+    cx<T>* nl;             // ROW_STAGE / ROW_R2C output (mixed)
+    double* out_ref[4];    // ROW_C2R outputs (FP64 [ny][nx]) if non-null
+    T* out[4];             // ROW_C2R outputs (T [ny][nx]) otherwise
+    const double* in_ref;  // ROW_R2C input (FP64 [ny][nx])
+    DevScalars* sc;
+};
+
+// Build the Hermitian extension of Z = A + iB in LDS for one row y from
+// the kept mixed columns (c2r convention: imaginary parts of the kx = 0
+// values are ignored, as FFTW's c2r does), then read the standard pattern.
+template <typename T>
+__device__ __forceinline__ void build_row_pair(cx<T> (&v)[16], cx<T>* lds, int j, const Line& g,
+                                               const Geo& G, const cx<T>* A, const cx<T>* B,
+                                               int y, bool active) {
+    const cx<T> zero = mk(T(0), T(0));
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int k = j + s * g.TPL;
+        if (k <= G.kxmax) {
+            cx<T> a = zero, b = zero;
+            if (active) {
+                a = A ? A[(size_t)k * G.ny + y] : zero;
+                b = B ? B[(size_t)k * G.ny + y] : zero;
+            }
+            if (k == 0) {
+                a.y = T(0);
+                b.y = T(0);
+            }
+            lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
+            if (k > 0) lds[pad(g.L - k)] = mk(a.x + b.y, b.x - a.y);
+        } else if (k < g.L - G.kxmax) {
+            lds[pad(k)] = zero;
+        }
+    }
+    __syncthreads();
+    lds_read_std(v, lds, j, g.TPL);
+}
+
+// Forward x-transform of two real rows packed as v = ra + i rb, then
+// separation of the two half spectra and store of the kept columns into
+// the mixed layout at rows y and y+1.
+template <typename T>
+__device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[16], cx<T>* lds, int j, const Line& g,
+                                               const Geo& G, const cx<T>* tw, cx<T>* out, int y,
+                                               bool active) {
+    line_fft<-1>(v, lds, j, g, tw);
+    lds_write_std(v, lds, j, g.TPL);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int k = j + s * g.TPL;
+        if (k <= G.kxmax && active) {
+            const cx<T> zk = lds[pad(k)];
+            const cx<T> zm = lds[pad((g.L - k) & (g.L - 1))];
+            const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
+            const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
+            out[(size_t)k * G.ny + y] = ja;
+            out[(size_t)k * G.ny + y + 1] = jb;
+        }
+    }
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_row(RowArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
+    __shared__ double red[NT / 64];
+    const Geo& G = a.geo;
+    const Line g = make_line(G.nx);
+    const int lines = NT / g.TPL;
+    const int m = threadIdx.x / g.TPL, j = threadIdx.x - m * g.TPL;
+    cx<T>* lds = smem + m * (g.L + g.L / 16);
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
+    cx<T> v[16];
+
+    if (a.mode == ROW_C2R) {
+        const int y = item;
+        const bool active = y < G.ny;
+        for (int f = 0; f < a.nfields; f += 2) {
+            const cx<T>* B = (f + 1 < a.nfields) ? a.mixed[f + 1] : nullptr;
+            build_row_pair(v, lds, j, g, G, a.mixed[f], B, y, active);
+            line_fft<+1>(v, lds, j, g, a.tw);
+            if (active) {
+#pragma unroll
+                for (int s = 0; s < 16; ++s) {
+                    const size_t o = (size_t)y * G.nx + j + s * g.TPL;
+                    if (a.out_ref[f]) a.out_ref[f][o] = double(v[s].x);
+                    else a.out[f][o] = v[s].x;
+                    if (B) {
+                        if (a.out_ref[f + 1]) a.out_ref[f + 1][o] = double(v[s].y);
+                        else a.out[f + 1][o] = v[s].y;
+                    }
+                }
+            }
+        }
+        return;
+    }
+
+    const int y0 = 2 * item;
+    const bool active = y0 < G.ny;
+
+    if (a.mode == ROW_R2C) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int x = j + s * g.TPL;
+            v[s] = active ? mk(T(a.in_ref[(size_t)y0 * G.nx + x]),
+                               T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]))
+                          : mk(T(0), T(0));
+        }
+        fwd_rows_store(v, lds, j, g, G, a.tw, a.nl, y0, active);
+        return;
+    }
+
+    // ROW_STAGE: J = psi_x omega_y - psi_y omega_x (spectral_core.c:101-104)
+    T jr[2][16];
+    double u2max = 0.0;
+    for (int r = 0; r < 2; ++r) {
+        const int y = y0 + r;
+        cx<T> z1[16];
+        build_row_pair(v, lds, j, g, G, a.mixed[0], a.mixed[3], y, active);  // psi_x + i omega_y
+        line_fft<+1>(v, lds, j, g, a.tw);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) z1[s] = v[s];
+        build_row_pair(v, lds, j, g, G, a.mixed[1], a.mixed[2], y, active);  // psi_y + i omega_x
+        line_fft<+1>(v, lds, j, g, a.tw);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            jr[r][s] = z1[s].x * z1[s].y - v[s].x * v[s].y;
+            double sp = (double)z1[s].x * z1[s].x + (double)v[s].x * v[s].x;
+            if (sp != sp) sp = __builtin_inf();  // NaN must win the max: divergence check
+            u2max = sp > u2max ? sp : u2max;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) v[s] = mk(jr[0][s], jr[1][s]);
+    fwd_rows_store(v, lds, j, g, G, a.tw, a.nl, y0, active);
+    if (a.track_umax) block_max_to(active ? u2max : 0.0, &a.sc->umax2_bits, red);
+}
+
+}  // namespace nsd
