@@ -1,0 +1,186 @@
+"""Throughput of the pseudospectral vorticity step on one MI355X.
+
+Workload (the 4096^2 forced-turbulence case the headline metric is quoted on):
+  2-D periodic [0, 2 pi]^2, N = 4096, nu = 1e-4 (Re = 1e4), del^8
+  hyperviscosity nu4 = 1e-16 applied as an integrating factor, linear drag
+  0.1, white-in-time forcing on the shell 4 <= |k| <= 6 at injection rate
+  1e-3 (counter SplitMix64, seed 42), adaptive dt at CFL 0.5, classical RK4
+  stages in FP32 (FP64 measured beside it).  Initial vorticity: random phases
+  with E(k) ~ k^4 exp(-(k/10)^2), unit energy, seed 42 (synthetic data).
+
+A step is one full RK4 step (4 x {row kernel, column kernel} + the forcing
+draw) over the whole grid, state resident in HBM.  The timed region launches
+the kernels eagerly with a HIP event between every two kernels on the
+solver's stream; the per-kernel averages give the roofline of the dominant
+kernel.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gpoint-steps/s (4096², FP32 vs FP64), % HBM peak, closure speedup@equal error"
+
+
+def workload(ns, n, fp32):
+    s = ns.Spectral(n, n, 2 * np.pi, 2 * np.pi, 1e4)
+    s.init(taylor_green=False)
+    kmax = np.sqrt(2.0) * n / 2.0  # hyperviscosity.c:68-73 (corner of the stored modes)
+    s.hv_set(enabled=1, coefficient=1e-16 * kmax ** 8, order=8, cutoff_ratio=0.0)
+    s.set_precision(fp32)
+    s.set_scheme(True)
+    s.set_drag(0.1)
+    s.set_forcing(4.0, 6.0, 1e-3, seed=42)
+    s.d.cfl_factor = 0.5
+    s.random_phase(42, k_peak=10.0, energy=1.0)
+    s.sync_device()
+    return s
+
+
+def kernel_bytes(n, fp32):
+    """Algorithmic HBM bytes per launch of the two stage kernels."""
+    nkx = n // 2 + 1
+    kxk = (2 * nkx) // 3 + 1            # kept kx columns
+    kyk = 2 * ((2 * n) // 6) + 1        # kept ky rows
+    esz = 8 if fp32 else 16             # complex element
+    M = kxk * n * esz                   # one mixed field [kx][y]
+    Cc = kxk * kyk * esz                # one compact spectral field
+    row = 5 * M                         # 4 mixed in, J out
+    col = 5 * M + 2.75 * Cc             # J in, 4 mixed out, state/acc traffic averaged
+    return row, col, M, Cc
+
+
+def time_gpu(ns, n, fp32, steps, warmup):
+    L = ns.load()
+    s = workload(ns, n, fp32)
+    ctx = s.ctx
+    if warmup:
+        s.check(L.nsd_advance_graph(ctx, warmup, 1))
+    s.check(L.nsd_synchronize(ctx))
+    # timed region: eager launches with per-kernel events
+    L.nsd_profile(ctx, 1)
+    t0 = time.perf_counter()
+    s.check(L.nsd_advance(ctx, steps, 1))
+    s.check(L.nsd_synchronize(ctx))
+    wall = time.perf_counter() - t0
+    L.nsd_profile(ctx, 0)
+    row_ms, col_ms, step_ms = C.c_double(), C.c_double(), C.c_double()
+    nst = C.c_int()
+    L.nsd_profile_read(ctx, C.byref(row_ms), C.byref(col_ms), C.byref(step_ms), C.byref(nst))
+    # the same steps replayed from the captured graph (host clock)
+    t0 = time.perf_counter()
+    s.check(L.nsd_advance_graph(ctx, steps, 1))
+    s.check(L.nsd_synchronize(ctx))
+    graph_wall = time.perf_counter() - t0
+    st = s.sync_host()
+    e, z = s.device_diagnostics()
+    res = dict(ms_per_step=step_ms.value, wall_ms_per_step=1e3 * wall / steps,
+               graph_ms_per_step=1e3 * graph_wall / steps, row_ms=row_ms.value,
+               col_ms=col_ms.value, t=s.d.t, dt=s.d.dt, energy=e, enstrophy=z, status=st)
+    s.close()
+    return res
+
+
+def cpu_baseline(n):
+    """Oracle (reference algorithm, project FFT, OpenMP) on this host: one step
+    of the same forced integrating-factor workload at FP64."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(n, n, 2 * np.pi, 2 * np.pi, 1e4)
+    o.init_grids()
+    kmax = np.sqrt(2.0) * n / 2.0
+    o.hv_set(enabled=1, coefficient=1e-16 * kmax ** 8, order=8, cutoff_ratio=0.0)
+    o.d.drag = 0.1
+    o.d.cfl_factor = 0.5
+    o.init_random_phase(42, 10.0, 1.0)
+    f = O.OracleForcing(o.k2, n // 2 + 1, n, 4.0, 6.0, energy_target=1.0, seed=42)
+    t0 = time.perf_counter()
+    o.d.dt = o.cfl()
+    o.advance_if(o.project_forcing(f.generate(o.d.dt), 1e-3))
+    t = time.perf_counter() - t0
+    return dict(value=n * n / t / 1e9, unit="Gpoint-steps/s", cores=threads, kind="port",
+                sample=f"1 RK4 step of the {n}^2 forced integrating-factor workload, FP64, "
+                       f"oracle with project FFT (not FFTW), {t:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as G
+    ns = G._import_package()
+    n = a.n
+    r32 = time_gpu(ns, n, True, a.steps, a.warmup)
+    r64 = None if a.no_fp64 else time_gpu(ns, n, False, max(a.steps // 2, 1), max(a.warmup // 2, 1))
+    value = n * n / (r32["ms_per_step"] * 1e-3) / 1e9
+
+    row_b, col_b, M, Cc = kernel_bytes(n, True)
+    dom = "k_col" if r32["col_ms"] >= r32["row_ms"] else "k_row"
+    dom_ms = max(r32["col_ms"], r32["row_ms"])
+    dom_b = col_b if dom == "k_col" else row_b
+    achieved = dom_b / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_fp32_{n}.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    step_bytes = 4 * (row_b + col_b)
+    survey_model = 64 * n * (n // 2 + 1) * 8  # 16 H-fields per stage at FP32
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpoint-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"forced 2-D turbulence {n}^2, FP32 stages, integrating-factor "
+                               "RK4, del^8 nu4=1e-16, drag 0.1, white forcing 4<=|k|<=6 rate "
+                               "1e-3 seed 42, adaptive dt CFL 0.5, nu=1e-4",
+                   "grid": n, "precision": "fp32"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes_per_launch": dom_b},
+        "step": {"row_ms": round(r32["row_ms"], 4), "col_ms": round(r32["col_ms"], 4),
+                 "graph_ms_per_step": round(r32["graph_ms_per_step"], 4),
+                 "algorithmic_bytes_per_step": step_bytes,
+                 "step_hbm_frac": round(step_bytes / (r32["ms_per_step"] * 1e-3) / 1e9
+                                        / HBM_PEAK_GBS, 4),
+                 "survey_model_hbm_frac": round(survey_model / (r32["ms_per_step"] * 1e-3) / 1e9
+                                                / HBM_PEAK_GBS, 4),
+                 "t": r32["t"], "dt": r32["dt"], "energy": r32["energy"], "status": r32["status"]},
+    }
+    if r64:
+        out["fp64"] = {"value": round(n * n / (r64["ms_per_step"] * 1e-3) / 1e9, 3),
+                       "ms_per_step": round(r64["ms_per_step"], 4),
+                       "row_ms": round(r64["row_ms"], 4), "col_ms": round(r64["col_ms"], 4)}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
