@@ -615,50 +615,94 @@ RowArgs<T> row_args(NsdContext* c) {
     return a;
 }
 
+template <typename T, int NT, int MODE, int SCHEME>
+void launch_col_k(NsdContext* c, const ColArgs<T>& a, int blocks, size_t lds) {
+    hipLaunchKernelGGL((k_col<T, NT, MODE, SCHEME>), dim3(blocks), dim3(NT), lds, c->stream, a);
+}
+
+template <typename T, int NT>
+void launch_col_nt(NsdContext* c, const ColArgs<T>& a, int blocks, size_t lds) {
+    switch (a.mode) {
+    case COL_STAGE:
+        if (a.phys.scheme == 0) launch_col_k<T, NT, COL_STAGE, 0>(c, a, blocks, lds);
+        else launch_col_k<T, NT, COL_STAGE, 1>(c, a, blocks, lds);
+        break;
+    case COL_PRIME: launch_col_k<T, NT, COL_PRIME, 0>(c, a, blocks, lds); break;
+    case COL_FWD: launch_col_k<T, NT, COL_FWD, 0>(c, a, blocks, lds); break;
+    default: launch_col_k<T, NT, COL_INV, 0>(c, a, blocks, lds); break;
+    }
+}
+
 template <typename T>
 void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
     const int L = c->G.ny;
-    const int NT = L >= 8192 ? 512 : 256;
-    const int lines = NT * 16 / L;
+    const int NT = threads_for(L);
+    const int lines = NT * P / L;
     const int blocks = (ncols + lines - 1) / lines;
     const size_t lds = (size_t)lines * (L + L / 16) * sizeof(cx<T>);
-    if (NT == 256)
-        hipLaunchKernelGGL((k_col<T, 256>), dim3(blocks), dim3(256), lds, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_col<T, 512>), dim3(blocks), dim3(512), lds, c->stream, a);
+    if (NT == 256) launch_col_nt<T, 256>(c, a, blocks, lds);
+    else if (NT == 512) launch_col_nt<T, 512>(c, a, blocks, lds);
+    else launch_col_nt<T, 1024>(c, a, blocks, lds);
+}
+
+template <typename T, int NT>
+void launch_row_nt(NsdContext* c, const RowArgs<T>& a, int blocks, size_t lds) {
+    switch (a.mode) {
+    case ROW_STAGE:
+        hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE>), dim3(blocks), dim3(NT), lds, c->stream, a);
+        break;
+    case ROW_C2R:
+        hipLaunchKernelGGL((k_row<T, NT, ROW_C2R>), dim3(blocks), dim3(NT), lds, c->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_row<T, NT, ROW_R2C>), dim3(blocks), dim3(NT), lds, c->stream, a);
+        break;
+    }
 }
 
 template <typename T>
 void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items) {
     const int L = c->G.nx;
-    const int NT = L >= 8192 ? 512 : 256;
-    const int lines = NT * 16 / L;
+    const int NT = threads_for(L);
+    const int lines = NT * P / L;
     const int blocks = (items + lines - 1) / lines;
     const size_t lds = (size_t)lines * (L + L / 16) * sizeof(cx<T>);
-    if (NT == 256)
-        hipLaunchKernelGGL((k_row<T, 256>), dim3(blocks), dim3(256), lds, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_row<T, 512>), dim3(blocks), dim3(512), lds, c->stream, a);
+    if (NT == 256) launch_row_nt<T, 256>(c, a, blocks, lds);
+    else if (NT == 512) launch_row_nt<T, 512>(c, a, blocks, lds);
+    else launch_row_nt<T, 1024>(c, a, blocks, lds);
 }
 
+template <typename K>
+void lds_attr(K kernel, int bytes) {
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              bytes);
+}
+
+template <typename T, int NT>
+void set_lds_limits_nt(int bytes) {
+    lds_attr(k_col<T, NT, COL_STAGE, 0>, bytes);
+    lds_attr(k_col<T, NT, COL_STAGE, 1>, bytes);
+    lds_attr(k_col<T, NT, COL_PRIME, 0>, bytes);
+    lds_attr(k_col<T, NT, COL_FWD, 0>, bytes);
+    lds_attr(k_col<T, NT, COL_INV, 0>, bytes);
+    lds_attr(k_row<T, NT, ROW_STAGE>, bytes);
+    lds_attr(k_row<T, NT, ROW_C2R>, bytes);
+    lds_attr(k_row<T, NT, ROW_R2C>, bytes);
+}
+
+// allow the large dynamic LDS footprints of the long lines (up to the
+// device's per-workgroup limit); failures surface at launch instead
 template <typename T>
 void set_lds_limits(int bytes) {
-    // allow the large dynamic LDS footprints of the long lines (up to the
-    // device's per-workgroup limit); failures surface at launch instead
-    (void)hipFuncSetAttribute((const void*)k_col<T, 256>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute((const void*)k_col<T, 512>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute((const void*)k_row<T, 256>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute((const void*)k_row<T, 512>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    set_lds_limits_nt<T, 256>(bytes);
+    set_lds_limits_nt<T, 512>(bytes);
+    set_lds_limits_nt<T, 1024>(bytes);
     (void)hipGetLastError();  // attribute calls must not leave a sticky error behind
 }
 
 size_t line_lds_bytes(int L, size_t esz) {
-    const int NT = L >= 8192 ? 512 : 256;
-    return (size_t)(NT * 16 / L) * (L + L / 16) * esz;
+    const int NT = threads_for(L);
+    return (size_t)(NT * P / L) * (L + L / 16) * esz;
 }
 
 // --- typed operations

@@ -1,27 +1,35 @@
 // nsd_fft.hpp -- workgroup-level Stockham FFT building blocks for gfx950.
 //
 // A "line" is one 1-D transform of length L (power of two, 16..8192) held by
-// TPL = L/16 threads, each thread owning 16 complex values.  Between passes
-// the line lives in LDS.  Thread j of a line always reads, and after the
-// last pass always holds, the elements n = j + s*TPL (s = 0..15): the
-// "standard pattern".  Because the first pass reads and the last pass
+// TPL = L/P threads, each thread owning P complex values (P = NSD_POINTS,
+// 8 by default).  Between passes the line lives in LDS.  Thread j of a line
+// always reads, and after the last pass always holds, the elements
+// n = j + s*TPL (s = 0..P-1): the "standard pattern".  Because the first pass reads and the last pass
 // writes that same pattern, pre- and post-processing (spectral derivatives,
 // masks, stage updates, pointwise products) are fused into the registers
 // between transforms without any extra data movement.
 //
-// Pass plan: floor(log16 L) radix-16 passes, then one radix-2/4/8 pass when
-// L is not a power of 16.  Pass with radix R and stride Ns (product of the
+// Pass plan: floor(log_P L) radix-P passes, then one radix-2/4/8 pass when
+// L is not a power of P.  Pass with radix R and stride Ns (product of the
 // previous radices): butterfly b reads b + r*L/R, multiplies by
 // W_{Ns R}^{(b mod Ns) r}, does an R-point DFT, and writes (b - b mod Ns)*R
 // + b mod Ns + r*Ns (Stockham autosort, natural-order output).
 // Twiddles come from a table W_L^m = exp(-2 pi i m / L), m < L, computed on
 // the host in FP64 with exact integer angle reduction (FP32 tables are the
-// FP64 values rounded once).
+// FP64 values rounded once); a radix-R butterfly loads W, W^2, W^4, W^8 and
+// forms the other powers by complex products.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#ifndef NSD_POINTS
+#define NSD_POINTS 8
+#endif
+
 namespace nsd {
+
+constexpr int P = NSD_POINTS;  // complex values per thread per line
+static_assert(P == 8 || P == 16, "points per thread must be 8 or 16");
 
 template <typename T>
 struct cx {
@@ -164,45 +172,63 @@ __device__ __forceinline__ void dft16(cx<T>* v) {
 // Line geometry shared by every kernel.
 struct Line {
     int L;    // transform length
-    int TPL;  // threads per line = L/16
-    int n16;  // number of radix-16 passes
+    int TPL;  // threads per line = L/P
+    int nP;   // number of radix-P passes
     int rem;  // trailing radix (1, 2, 4 or 8)
 };
 
 __host__ __device__ inline Line make_line(int L) {
     Line g;
     g.L = L;
-    g.TPL = L / 16;
-    int n16 = 0, r = L;
-    while (r >= 16 && (r % 16) == 0) {
-        r /= 16;
-        ++n16;
+    g.TPL = L / P;
+    int n = 0, r = L;
+    while (r >= P && (r % P) == 0) {
+        r /= P;
+        ++n;
     }
-    g.n16 = n16;
+    g.nP = n;
     g.rem = r;
     return g;
 }
 
-// One Stockham pass of radix R on the thread's 16 registers.
-// If `last`, the outputs stay in v (standard pattern); otherwise they are
-// written to the line's LDS region.
+// threads per workgroup for lines of length L (a workgroup holds whole lines)
+__host__ __device__ inline int threads_for(int L) { return L / P > 256 ? L / P : 256; }
+
+// One Stockham pass of radix R on the thread's P registers (Q = P/R
+// butterflies per thread).  If `last`, the outputs stay in v (standard
+// pattern); otherwise they are written to the line's LDS region.
 template <int DIR, int R, typename T>
-__device__ __forceinline__ void stockham_pass(cx<T> (&v)[16], int j, const Line& g, int Ns,
+__device__ __forceinline__ void stockham_pass(cx<T> (&v)[P], int j, const Line& g, int Ns,
                                               const cx<T>* __restrict__ tw, cx<T>* lds,
                                               bool last) {
-    constexpr int Q = 16 / R;
+    constexpr int Q = P / R;
     const int tstride = g.L / (Ns * R);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int b = j + q * g.TPL;
         const int k = b & (Ns - 1);
         if (Ns > 1) {
+            // powers W^r of W = W_{Ns R}^k: W, W^2, W^4, W^8 from the table,
+            // the others as products (at most three roundings each)
             const int step = k * tstride;
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const cx<T> w = tw[r * step];
-                v[q + r * Q] = DIR < 0 ? cmul(v[q + r * Q], w) : cmulc(v[q + r * Q], w);
+            cx<T> w[R];
+            w[1] = tw[step];
+            if constexpr (R > 2) w[2] = tw[2 * step];
+            if constexpr (R > 4) w[4] = tw[4 * step];
+            if constexpr (R > 8) w[8] = tw[8 * step];
+            if constexpr (R > 2) w[3] = cmul(w[1], w[2]);
+            if constexpr (R > 4) {
+                w[5] = cmul(w[1], w[4]);
+                w[6] = cmul(w[2], w[4]);
+                w[7] = cmul(w[3], w[4]);
             }
+            if constexpr (R > 8) {
+#pragma unroll
+                for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], w[8]);
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                v[q + r * Q] = DIR < 0 ? cmul(v[q + r * Q], w[r]) : cmulc(v[q + r * Q], w[r]);
         }
         if constexpr (R == 16) {
             dft16<DIR>(v);
@@ -223,36 +249,36 @@ __device__ __forceinline__ void stockham_pass(cx<T> (&v)[16], int j, const Line&
 }
 
 template <typename T>
-__device__ __forceinline__ void lds_read_std(cx<T> (&v)[16], const cx<T>* lds, int j, int TPL) {
+__device__ __forceinline__ void lds_read_std(cx<T> (&v)[P], const cx<T>* lds, int j, int TPL) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) v[s] = lds[pad(j + s * TPL)];
+    for (int s = 0; s < P; ++s) v[s] = lds[pad(j + s * TPL)];
 }
 
 template <typename T>
-__device__ __forceinline__ void lds_write_std(const cx<T> (&v)[16], cx<T>* lds, int j, int TPL) {
+__device__ __forceinline__ void lds_write_std(const cx<T> (&v)[P], cx<T>* lds, int j, int TPL) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) lds[pad(j + s * TPL)] = v[s];
+    for (int s = 0; s < P; ++s) lds[pad(j + s * TPL)] = v[s];
 }
 
 // Full line transform: v (standard pattern, in registers) -> v (standard
 // pattern, natural-order result).  Unnormalised.  DIR = -1 forward
 // (exp(-i...)), +1 inverse.  Every thread of the workgroup must call it
 // (it contains workgroup barriers); the line's LDS region must be free on
-// entry (a barrier is taken first).
+// entry (a barrier is taken first) and is free again on return.
 template <int DIR, typename T>
-__device__ __forceinline__ void line_fft(cx<T> (&v)[16], cx<T>* lds, int j, const Line& g,
+__device__ __forceinline__ void line_fft(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                          const cx<T>* __restrict__ tw) {
     int Ns = 1;
-    const int passes = g.n16 + (g.rem > 1 ? 1 : 0);
+    const int passes = g.nP + (g.rem > 1 ? 1 : 0);
     for (int p = 0; p < passes; ++p) {
         const bool last = (p == passes - 1);
         if (p > 0) {
             lds_read_std(v, lds, j, g.TPL);
         }
         __syncthreads();  // reads of the previous pass (or earlier users) done
-        if (p < g.n16) {
-            stockham_pass<DIR, 16>(v, j, g, Ns, tw, lds, last);
-            Ns *= 16;
+        if (p < g.nP) {
+            stockham_pass<DIR, P>(v, j, g, Ns, tw, lds, last);
+            Ns *= P;
         } else if (g.rem == 8) {
             stockham_pass<DIR, 8>(v, j, g, Ns, tw, lds, last);
         } else if (g.rem == 4) {

@@ -101,6 +101,20 @@ __device__ __forceinline__ double sfd_weight(const Phys& p, double kmag) {
     return p.sfd_strength * powi(s, p.hv_order);
 }
 
+// the same weights evaluated at the working precision of the stage kernels
+template <typename T>
+__device__ __forceinline__ T hv_weight_t(const Phys& p, T kmag) {
+    if (!p.hv_on || kmag <= T(p.hv_kc)) return T(0);
+    return T(p.hv_coeff) * powi((kmag - T(p.hv_kc)) * T(p.hv_invbw), p.hv_order);
+}
+template <typename T>
+__device__ __forceinline__ T sfd_weight_t(const Phys& p, T kmag) {
+    if (!p.sfd_on || p.hv_invbw <= 0.0 || kmag <= T(p.hv_kc)) return T(0);
+    T s = (kmag - T(p.hv_kc)) * T(p.hv_invbw);
+    s = s < T(0) ? T(0) : (s > T(1) ? T(1) : s);
+    return T(p.sfd_strength) * powi(s, p.hv_order);
+}
+
 // spectral_time_integration.c:22-28 with MIN_DT/MAX_DT of common_types.h:32-33
 __device__ __forceinline__ double cfl_dt(const Phys& p, double umax) {
     const double dt_cfl = p.cfl * p.dxmin / (umax + 1e-12);
@@ -235,8 +249,12 @@ __device__ __forceinline__ cx<T> apply_op(int op, cx<T> a, cx<T> b, T kx, T ky, 
     return mul_i(psi, -kx);  // OP_V0
 }
 
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
+// occupancy floor handed to the register allocator: waves per SIMD
+template <typename T>
+constexpr int min_waves() { return sizeof(T) == 4 ? 4 : 2; }
+
+template <typename T, int NT, int MODE, int SCHEME>
+__global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     const Geo& G = a.geo;
@@ -245,28 +263,28 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
     const int m = threadIdx.x / g.TPL, j = threadIdx.x - m * g.TPL;
     cx<T>* lds = smem + m * (g.L + g.L / 16);
     const int kxi = blockIdx.x * lines + m;
-    const int ncols = (a.mode == COL_FWD) ? G.nkx : G.nkx_keep;
+    const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
     const bool active = kxi < ncols;
     const bool has_data = active && kxi < G.nkx_keep;
-    const T kx = T(kx_of(G, kxi));
+    const T kx = T(kxi) * T(2.0 * kPi / G.Lx);
     const size_t mbase = (size_t)kxi * G.ny;
     const cx<T> zero = mk(T(0), T(0));
 
-    cx<T> v[16];
-    cx<T> w[16];
+    cx<T> v[P];
+    cx<T> w[P];
 
     // ---- forward y-transform of the mixed nonlinear field
-    if (a.mode == COL_STAGE || a.mode == COL_FWD) {
+    if constexpr (MODE == COL_STAGE || MODE == COL_FWD) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) v[s] = has_data ? a.nl[mbase + j + s * g.TPL] : zero;
+        for (int s = 0; s < P; ++s) v[s] = has_data ? a.nl[mbase + j + s * g.TPL] : zero;
         line_fft<-1>(v, lds, j, g, a.tw);
     }
 
-    if (a.mode == COL_FWD) {
+    if constexpr (MODE == COL_FWD) {
         // spectral_transforms.c:13-21: x 1/(nx ny), then zero outside the mask
         const T norm = T(a.fwd_scale / ((double)G.nx * (double)G.ny));
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
+        for (int s = 0; s < P; ++s) {
             const int jy = j + s * g.TPL;
             cx<T> r = (has_data && kept(G, kxi, jy)) ? norm * v[s] : zero;
             if (active) st_ref(a.out_ref, (size_t)jy * G.nkx + kxi, r);
@@ -276,7 +294,7 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
 
     // ---- spectral state of this column after the stage (or as is)
     double dt = 0.0;
-    if (a.mode == COL_STAGE) {
+    if constexpr (MODE == COL_STAGE) {
         if (a.stage == 0 && a.phys.adaptive) {
             const double u2 = __longlong_as_double((long long)a.sc->umax2_bits);
             dt = cfl_dt(a.phys, sqrt(u2));
@@ -294,33 +312,33 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
     const T sqh = T(sqrt(dt));
 
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
+    for (int s = 0; s < P; ++s) {
         const int jy = j + s * g.TPL;
         w[s] = zero;
         if (!(has_data && kept(G, kxi, jy))) continue;
         const size_t ci = (size_t)kxi * G.nky_keep + compact_y(G, jy);
-        const T ky = T(ky_of(G, jy));
-        if (a.mode == COL_PRIME) {
+        const T ky = T(band_of(G, jy)) * T(2.0 * kPi / G.Ly);
+        if constexpr (MODE == COL_PRIME) {
             w[s] = a.state[ci];
             continue;
         }
-        if (a.mode == COL_INV) {
+        if constexpr (MODE == COL_INV) {
             w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : a.in0[ci];
             continue;
         }
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
         const cx<T> N = (-norm) * v[s];
         const cx<T> w0 = a.state[ci];
-        const double k2d = (double)kx * kx + (double)ky * ky;
-        const double kmag = sqrt(k2d);
+        const T k2 = kx * kx + ky * ky;
+        const T kmag = sqrt(k2);
         cx<T> wn;
-        if (a.phys.scheme == 0) {
+        if constexpr (SCHEME == 0) {
             // classic RK4 (spectral_time_integration.c:50-100) with
             // hyperviscosity_apply (hyperviscosity.c:104-150) and drag
             const cx<T> ws = (a.stage == 0) ? w0 : a.stagebuf[ci];
-            const T lin = T(-(k2d * a.phys.inv_Re + a.phys.drag + hv_weight(a.phys, kmag)));
+            const T lin = -(k2 * T(a.phys.inv_Re) + T(a.phys.drag) + hv_weight_t<T>(a.phys, kmag));
             cx<T> ks = N + lin * ws;
-            const T sw = T(sfd_weight(a.phys, kmag));
+            const T sw = sfd_weight_t<T>(a.phys, kmag);
             if (sw > T(0)) ks = ks - sw * (ws - a.sfd[ci]);
             if (a.stage == 0) {
                 a.acc[ci] = w0 + (h / T(6)) * ks;
@@ -341,8 +359,8 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
             if (a.stage < 3) a.stagebuf[ci] = wn;
         } else {
             // integrating factor, L = -(k^2/Re + w_hv + mu)
-            const double L = -(k2d * a.phys.inv_Re + a.phys.drag + hv_weight(a.phys, kmag));
-            const T E2 = T(exp(L * 0.5 * dt));
+            const T L = -(k2 * T(a.phys.inv_Re) + T(a.phys.drag) + hv_weight_t<T>(a.phys, kmag));
+            const T E2 = exp(L * (T(0.5) * h));
             const T E = E2 * E2;
             if (a.stage == 0) {
                 a.acc[ci] = E * w0 + (h / T(6) * E) * N;
@@ -368,39 +386,59 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs<T> a) {
         w[s] = wn;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (a.mode == COL_STAGE && a.stage == 3) {
+        if (MODE == COL_STAGE && a.stage == 3) {
             a.sc->t += dt;
             a.sc->umax2_bits = 0ull;
-        } else if (a.mode == COL_PRIME) {
+        } else if (MODE == COL_PRIME) {
             a.sc->umax2_bits = 0ull;
         }
     }
 
     // ---- derivative spectra and inverse y-transforms
-    const int nout = (a.mode == COL_INV) ? a.nout : 4;
-    for (int f = 0; f < nout; ++f) {
+    const T kys = T(2.0 * kPi / G.Ly);
+    if constexpr (MODE == COL_INV) {
+        for (int f = 0; f < a.nout; ++f) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int jy = j + s * g.TPL;
-            const T ky = T(ky_of(G, jy));
-            const T k2 = kx * kx + ky * ky;
-            if (a.mode == COL_INV) {
+            for (int s = 0; s < P; ++s) {
+                const int jy = j + s * g.TPL;
+                const T ky = T(band_of(G, jy)) * kys;
+                const T k2 = kx * kx + ky * ky;
                 cx<T> b = zero;
                 if (a.in1_ref && has_data && kept(G, kxi, jy))
                     b = ld_ref<T>(a.in1_ref, (size_t)jy * G.nkx + kxi);
                 v[s] = apply_op(a.ops[f], w[s], b, kx, ky, k2);
-            } else {
-                // f: 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99)
-                const cx<T> psi = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
-                const cx<T> src = (f < 2) ? psi : w[s];
-                v[s] = mul_i(src, (f & 1) ? ky : kx);
+            }
+            line_fft<+1>(v, lds, j, g, a.tw);
+            if (has_data) {
+                cx<T>* out = a.mixed[f];
+#pragma unroll
+                for (int s = 0; s < P; ++s) out[mbase + j + s * g.TPL] = v[s];
             }
         }
-        line_fft<+1>(v, lds, j, g, a.tw);
-        if (has_data) {
-            cx<T>* out = a.mixed[f];
+    } else {
+        // outputs 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99);
+        // the omega-based pair first, then w becomes psi_hat = -w/k^2 in place
+#pragma unroll 1
+        for (int f = 3; f >= 0; --f) {
+            if (f == 1) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) out[mbase + j + s * g.TPL] = v[s];
+                for (int s = 0; s < P; ++s) {
+                    const T ky = T(band_of(G, j + s * g.TPL)) * kys;
+                    const T k2 = kx * kx + ky * ky;
+                    w[s] = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const T ky = T(band_of(G, j + s * g.TPL)) * kys;
+                v[s] = mul_i(w[s], (f & 1) ? ky : kx);
+            }
+            line_fft<+1>(v, lds, j, g, a.tw);
+            if (has_data) {
+                cx<T>* out = a.mixed[f];
+#pragma unroll
+                for (int s = 0; s < P; ++s) out[mbase + j + s * g.TPL] = v[s];
+            }
         }
     }
 }
@@ -428,13 +466,13 @@ struct RowArgs {
 // the kept mixed columns (c2r convention: imaginary parts of the kx = 0
 // values are ignored, as FFTW's c2r does), then read the standard pattern.
 template <typename T>
-__device__ __forceinline__ void build_row_pair(cx<T> (&v)[16], cx<T>* lds, int j, const Line& g,
+__device__ __forceinline__ void build_row_pair(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                                const Geo& G, const cx<T>* A, const cx<T>* B,
                                                int y, bool active) {
     const cx<T> zero = mk(T(0), T(0));
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
+    for (int s = 0; s < P; ++s) {
         const int k = j + s * g.TPL;
         if (k <= G.kxmax) {
             cx<T> a = zero, b = zero;
@@ -456,18 +494,51 @@ __device__ __forceinline__ void build_row_pair(cx<T> (&v)[16], cx<T>* lds, int j
     lds_read_std(v, lds, j, g.TPL);
 }
 
+// Same Hermitian extension for Z = F(., y) + i F(., y+1): the two rows of one
+// field, fetched with a single 2-element load per column (the rows are
+// adjacent in the [kx][y] mixed layout).
+template <typename T>
+__device__ __forceinline__ void build_two_rows(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
+                                               const Geo& G, const cx<T>* F, int y,
+                                               bool active) {
+    const cx<T> zero = mk(T(0), T(0));
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+        const int k = j + s * g.TPL;
+        if (k <= G.kxmax) {
+            cx<T> a = zero, b = zero;
+            if (active) {
+                const cx<T>* src = F + (size_t)k * G.ny + y;
+                a = src[0];
+                b = src[1];
+            }
+            if (k == 0) {
+                a.y = T(0);
+                b.y = T(0);
+            }
+            lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
+            if (k > 0) lds[pad(g.L - k)] = mk(a.x + b.y, b.x - a.y);
+        } else if (k < g.L - G.kxmax) {
+            lds[pad(k)] = zero;
+        }
+    }
+    __syncthreads();
+    lds_read_std(v, lds, j, g.TPL);
+}
+
 // Forward x-transform of two real rows packed as v = ra + i rb, then
 // separation of the two half spectra and store of the kept columns into
 // the mixed layout at rows y and y+1.
 template <typename T>
-__device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[16], cx<T>* lds, int j, const Line& g,
+__device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                                const Geo& G, const cx<T>* tw, cx<T>* out, int y,
                                                bool active) {
     line_fft<-1>(v, lds, j, g, tw);
     lds_write_std(v, lds, j, g.TPL);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
+    for (int s = 0; s < P; ++s) {
         const int k = j + s * g.TPL;
         if (k <= G.kxmax && active) {
             const cx<T> zk = lds[pad(k)];
@@ -480,8 +551,8 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[16], cx<T>* lds, int j
     }
 }
 
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_row(RowArgs<T> a) {
+template <typename T, int NT, int MODE>
+__global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     __shared__ double red[NT / 64];
@@ -492,9 +563,9 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs<T> a) {
     cx<T>* lds = smem + m * (g.L + g.L / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
-    cx<T> v[16];
+    cx<T> v[P];
 
-    if (a.mode == ROW_C2R) {
+    if constexpr (MODE == ROW_C2R) {
         const int y = item;
         const bool active = y < G.ny;
         for (int f = 0; f < a.nfields; f += 2) {
@@ -503,7 +574,7 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs<T> a) {
             line_fft<+1>(v, lds, j, g, a.tw);
             if (active) {
 #pragma unroll
-                for (int s = 0; s < 16; ++s) {
+                for (int s = 0; s < P; ++s) {
                     const size_t o = (size_t)y * G.nx + j + s * g.TPL;
                     if (a.out_ref[f]) a.out_ref[f][o] = double(v[s].x);
                     else a.out[f][o] = v[s].x;
@@ -520,9 +591,9 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs<T> a) {
     const int y0 = 2 * item;
     const bool active = y0 < G.ny;
 
-    if (a.mode == ROW_R2C) {
+    if constexpr (MODE == ROW_R2C) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
+        for (int s = 0; s < P; ++s) {
             const int x = j + s * g.TPL;
             v[s] = active ? mk(T(a.in_ref[(size_t)y0 * G.nx + x]),
                                T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]))
@@ -532,28 +603,38 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs<T> a) {
         return;
     }
 
-    // ROW_STAGE: J = psi_x omega_y - psi_y omega_x (spectral_core.c:101-104)
-    T jr[2][16];
+    // ROW_STAGE: J = psi_x omega_y - psi_y omega_x (spectral_core.c:101-104).
+    // Each field is transformed for both rows at once (f(y0) + i f(y0+1)),
+    // in the order psi_x, psi_y, omega_x, omega_y, so that only two packed
+    // accumulators live across transforms.
+    cx<T> X[P], Jp[P];
     double u2max = 0.0;
-    for (int r = 0; r < 2; ++r) {
-        const int y = y0 + r;
-        cx<T> z1[16];
-        build_row_pair(v, lds, j, g, G, a.mixed[0], a.mixed[3], y, active);  // psi_x + i omega_y
-        line_fft<+1>(v, lds, j, g, a.tw);
+    build_two_rows(v, lds, j, g, G, a.mixed[0], y0, active);  // psi_x
+    line_fft<+1>(v, lds, j, g, a.tw);
 #pragma unroll
-        for (int s = 0; s < 16; ++s) z1[s] = v[s];
-        build_row_pair(v, lds, j, g, G, a.mixed[1], a.mixed[2], y, active);  // psi_y + i omega_x
-        line_fft<+1>(v, lds, j, g, a.tw);
+    for (int s = 0; s < P; ++s) X[s] = v[s];
+    build_two_rows(v, lds, j, g, G, a.mixed[1], y0, active);  // psi_y
+    line_fft<+1>(v, lds, j, g, a.tw);
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            jr[r][s] = z1[s].x * z1[s].y - v[s].x * v[s].y;
-            double sp = (double)z1[s].x * z1[s].x + (double)v[s].x * v[s].x;
-            if (sp != sp) sp = __builtin_inf();  // NaN must win the max: divergence check
-            u2max = sp > u2max ? sp : u2max;
-        }
+    for (int s = 0; s < P; ++s) {
+        // CFL speed: u = psi_y, v = -psi_x (spectral_core.c:30-31)
+        double sa = (double)X[s].x * X[s].x + (double)v[s].x * v[s].x;
+        double sb = (double)X[s].y * X[s].y + (double)v[s].y * v[s].y;
+        if (sa != sa) sa = __builtin_inf();  // NaN must win the max: divergence check
+        if (sb != sb) sb = __builtin_inf();
+        u2max = sa > u2max ? sa : u2max;
+        u2max = sb > u2max ? sb : u2max;
+        Jp[s] = v[s];  // psi_y, folded with omega_x next
     }
+    build_two_rows(v, lds, j, g, G, a.mixed[2], y0, active);  // omega_x
+    line_fft<+1>(v, lds, j, g, a.tw);
 #pragma unroll
-    for (int s = 0; s < 16; ++s) v[s] = mk(jr[0][s], jr[1][s]);
+    for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
+    build_two_rows(v, lds, j, g, G, a.mixed[3], y0, active);  // omega_y
+    line_fft<+1>(v, lds, j, g, a.tw);
+#pragma unroll
+    for (int s = 0; s < P; ++s)
+        v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
     fwd_rows_store(v, lds, j, g, G, a.tw, a.nl, y0, active);
     if (a.track_umax) block_max_to(active ? u2max : 0.0, &a.sc->umax2_bits, red);
 }
