@@ -58,10 +58,18 @@ def kernel_bytes(n, fp32):
     return row, col, M, Cc
 
 
-def time_gpu(ns, n, fp32, steps, warmup):
+PROBE_BYTES = 1 << 30
+
+
+def time_gpu(ns, n, fp32, steps, warmup, probe=False):
     L = ns.load()
     s = workload(ns, n, fp32)
     ctx = s.ctx
+    probe_gbs = None
+    if probe:
+        ms = C.c_double()
+        s.check(L.nsd_bandwidth_probe(ctx, PROBE_BYTES, 5, C.byref(ms)))
+        probe_gbs = 2 * PROBE_BYTES / (ms.value * 1e-3) / 1e9
     if warmup:
         s.check(L.nsd_advance_graph(ctx, warmup, 1))
     s.check(L.nsd_synchronize(ctx))
@@ -84,7 +92,8 @@ def time_gpu(ns, n, fp32, steps, warmup):
     e, z = s.device_diagnostics()
     res = dict(ms_per_step=step_ms.value, wall_ms_per_step=1e3 * wall / steps,
                graph_ms_per_step=1e3 * graph_wall / steps, row_ms=row_ms.value,
-               col_ms=col_ms.value, t=s.d.t, dt=s.d.dt, energy=e, enstrophy=z, status=st)
+               col_ms=col_ms.value, t=s.d.t, dt=s.d.dt, energy=e, enstrophy=z, status=st,
+               probe_gbs=probe_gbs)
     s.close()
     return res
 
@@ -120,6 +129,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="also time a plain 1 GiB device copy (k_copy) for counter calibration")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -129,7 +140,7 @@ def main():
     import __graft_entry__ as G
     ns = G._import_package()
     n = a.n
-    r32 = time_gpu(ns, n, True, a.steps, a.warmup)
+    r32 = time_gpu(ns, n, True, a.steps, a.warmup, a.probe)
     r64 = None if a.no_fp64 else time_gpu(ns, n, False, max(a.steps // 2, 1), max(a.warmup // 2, 1))
     value = n * n / (r32["ms_per_step"] * 1e-3) / 1e9
 
@@ -173,6 +184,8 @@ def main():
                                                 / HBM_PEAK_GBS, 4),
                  "t": r32["t"], "dt": r32["dt"], "energy": r32["energy"], "status": r32["status"]},
     }
+    if r32["probe_gbs"]:
+        out["copy_probe_gbs"] = round(r32["probe_gbs"], 1)
     if r64:
         out["fp64"] = {"value": round(n * n / (r64["ms_per_step"] * 1e-3) / 1e9, 3),
                        "ms_per_step": round(r64["ms_per_step"], 4),

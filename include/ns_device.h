@@ -80,6 +80,11 @@ int nsd_profile(NsdContext *c, int on);
 int nsd_profile_read(NsdContext *c, double *row_ms, double *col_ms, double *step_ms,
                      int *steps);
 
+/* copy `bytes` between two fresh device buffers `reps` times with a plain
+ * streaming kernel (k_copy); *ms = average duration of one copy.  Used to
+ * measure attainable bandwidth and to calibrate HBM byte counters. */
+int nsd_bandwidth_probe(NsdContext *c, size_t bytes, int reps, double *ms);
+
 /* ---- compat operations on host arrays (reference layouts, FP64) */
 int nsd_grids(NsdContext *c, double *x, double *y, double *kx, double *ky, double *k2, int *mask);
 int nsd_hv_weights(NsdContext *c, int enabled, double coefficient, int order, double cutoff_ratio,

@@ -62,7 +62,7 @@ EXPORTED = [
     "nsd_hv_weights", "nsd_physical_to_spectral", "nsd_spectral_to_physical", "nsd_dealias",
     "nsd_velocity", "nsd_derivatives", "nsd_nonlinear", "nsd_hv_apply", "nsd_hv_post_step",
     "nsd_taylor_green", "nsd_phys_reduce", "nsd_spec_reduce", "nsd_spectrum",
-    "nsd_forcing_generate",
+    "nsd_forcing_generate", "nsd_bandwidth_probe",
 ]
 
 
@@ -216,6 +216,8 @@ def load():
     L.nsd_advance.restype = C.c_int
     L.nsd_advance_graph.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.nsd_advance_graph.restype = C.c_int
+    L.nsd_bandwidth_probe.argtypes = [C.c_void_p, C.c_size_t, C.c_int, c_double_p]
+    L.nsd_bandwidth_probe.restype = C.c_int
     L.nsd_synchronize.argtypes = [C.c_void_p]
     L.nsd_synchronize.restype = C.c_int
     _lib = L

@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "../../../include/ns_device.h"
-#include "nsd_kernels.hpp"
+#include "nsd_stage.hpp"
 
 using namespace nsd;
 
@@ -24,6 +24,13 @@ __global__ void k_fill(double* p, size_t n, double v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// plain streaming copy (bandwidth probe and counter calibration)
+__global__ void k_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
 }
 
 // reference layout (FP64, full H) -> compact state (kept modes)
@@ -615,27 +622,23 @@ RowArgs<T> row_args(NsdContext* c) {
     return a;
 }
 
-template <typename T, int NT, int MODE, int SCHEME>
-void launch_col_k(NsdContext* c, const ColArgs<T>& a, int blocks, size_t lds) {
-    hipLaunchKernelGGL((k_col<T, NT, MODE, SCHEME>), dim3(blocks), dim3(NT), lds, c->stream, a);
-}
-
 template <typename T, int NT>
 void launch_col_nt(NsdContext* c, const ColArgs<T>& a, int blocks, size_t lds) {
-    switch (a.mode) {
-    case COL_STAGE:
-        if (a.phys.scheme == 0) launch_col_k<T, NT, COL_STAGE, 0>(c, a, blocks, lds);
-        else launch_col_k<T, NT, COL_STAGE, 1>(c, a, blocks, lds);
-        break;
-    case COL_PRIME: launch_col_k<T, NT, COL_PRIME, 0>(c, a, blocks, lds); break;
-    case COL_FWD: launch_col_k<T, NT, COL_FWD, 0>(c, a, blocks, lds); break;
-    default: launch_col_k<T, NT, COL_INV, 0>(c, a, blocks, lds); break;
-    }
+    if (a.mode == COL_FWD)
+        hipLaunchKernelGGL((k_col<T, NT, COL_FWD, 0, 0>), dim3(blocks), dim3(NT), lds, c->stream,
+                           a);
+    else
+        hipLaunchKernelGGL((k_col<T, NT, COL_INV, 0, 0>), dim3(blocks), dim3(NT), lds, c->stream,
+                           a);
 }
 
 template <typename T>
 void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
     const int L = c->G.ny;
+    if (a.mode == COL_STAGE || a.mode == COL_PRIME) {
+        launch_col_fused<T>(a, L, ncols, c->stream);
+        return;
+    }
     const int NT = threads_for(L);
     const int lines = NT * P / L;
     const int blocks = (ncols + lines - 1) / lines;
@@ -647,22 +650,19 @@ void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
 
 template <typename T, int NT>
 void launch_row_nt(NsdContext* c, const RowArgs<T>& a, int blocks, size_t lds) {
-    switch (a.mode) {
-    case ROW_STAGE:
-        hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE>), dim3(blocks), dim3(NT), lds, c->stream, a);
-        break;
-    case ROW_C2R:
-        hipLaunchKernelGGL((k_row<T, NT, ROW_C2R>), dim3(blocks), dim3(NT), lds, c->stream, a);
-        break;
-    default:
-        hipLaunchKernelGGL((k_row<T, NT, ROW_R2C>), dim3(blocks), dim3(NT), lds, c->stream, a);
-        break;
-    }
+    if (a.mode == ROW_C2R)
+        hipLaunchKernelGGL((k_row<T, NT, ROW_C2R, 0>), dim3(blocks), dim3(NT), lds, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_row<T, NT, ROW_R2C, 0>), dim3(blocks), dim3(NT), lds, c->stream, a);
 }
 
 template <typename T>
 void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items) {
     const int L = c->G.nx;
+    if (a.mode == ROW_STAGE) {
+        launch_row_stage<T>(a, L, items, c->stream);
+        return;
+    }
     const int NT = threads_for(L);
     const int lines = NT * P / L;
     const int blocks = (items + lines - 1) / lines;
@@ -680,14 +680,10 @@ void lds_attr(K kernel, int bytes) {
 
 template <typename T, int NT>
 void set_lds_limits_nt(int bytes) {
-    lds_attr(k_col<T, NT, COL_STAGE, 0>, bytes);
-    lds_attr(k_col<T, NT, COL_STAGE, 1>, bytes);
-    lds_attr(k_col<T, NT, COL_PRIME, 0>, bytes);
-    lds_attr(k_col<T, NT, COL_FWD, 0>, bytes);
-    lds_attr(k_col<T, NT, COL_INV, 0>, bytes);
-    lds_attr(k_row<T, NT, ROW_STAGE>, bytes);
-    lds_attr(k_row<T, NT, ROW_C2R>, bytes);
-    lds_attr(k_row<T, NT, ROW_R2C>, bytes);
+    lds_attr(k_col<T, NT, COL_FWD, 0, 0>, bytes);
+    lds_attr(k_col<T, NT, COL_INV, 0, 0>, bytes);
+    lds_attr(k_row<T, NT, ROW_C2R, 0>, bytes);
+    lds_attr(k_row<T, NT, ROW_R2C, 0>, bytes);
 }
 
 // allow the large dynamic LDS footprints of the long lines (up to the
@@ -697,6 +693,7 @@ void set_lds_limits(int bytes) {
     set_lds_limits_nt<T, 256>(bytes);
     set_lds_limits_nt<T, 512>(bytes);
     set_lds_limits_nt<T, 1024>(bytes);
+    stage_lds_limits<T>(bytes);
     (void)hipGetLastError();  // attribute calls must not leave a sticky error behind
 }
 
@@ -1265,6 +1262,38 @@ int nsd_profile_read(NsdContext* c, double* row_ms, double* col_ms, double* step
     if (step_ms) *step_ms = c->prof_step;
     if (steps) *steps = c->prof_steps;
     return ST_OK;
+}
+
+int nsd_bandwidth_probe(NsdContext* c, size_t bytes, int reps, double* ms) {
+    if (!c || bytes < 16 || reps < 1) return ST_PARAM;
+    const size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    CK(hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        c->err = "bandwidth probe allocation failed";
+        return ST_MEM;
+    }
+    CK(hipMemsetAsync(a, 0, n * 16, c->stream));
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    const unsigned grid = 256 * 32;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n);
+    CK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)a,
+                           (float4*)b, n);
+    CK(hipEventRecord(e1, c->stream));
+    CK(hipEventSynchronize(e1));
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return lastk(c);
 }
 
 int nsd_grids(NsdContext* c, double* x, double* y, double* kx, double* ky, double* k2, int* mask) {

@@ -290,6 +290,131 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[P], cx<T>* lds, int j, const
     }
 }
 
+// ------------------------------------------------------------------------
+// Compile-time lines: the same Stockham plan with L, the radix of every
+// pass and every stride known to the compiler, so that LDS addresses and
+// twiddle indices fold into immediate offsets.  Used by the time-step
+// kernels, which run at a fixed size for the whole simulation.
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+constexpr int ipowc(int b, int e) { return e == 0 ? 1 : b * ipowc(b, e - 1); }
+
+template <int L>
+struct CLine {
+    static constexpr int TPL = L / P;
+    static constexpr int nP = ilog2c(L) / ilog2c(P);
+    static constexpr int rem = L / ipowc(P, nP);
+    static constexpr int passes = nP + (rem > 1 ? 1 : 0);
+    static constexpr int radix(int p) { return p < nP ? P : rem; }
+    static constexpr int stride(int p) { return p == 0 ? 1 : stride(p - 1) * radix(p - 1); }
+};
+
+// LDS offset of element i under the one-in-sixteen padding
+constexpr int padc(int i) { return i + (i >> 4); }
+
+template <int L, typename T>
+__device__ __forceinline__ void lds_read_c(cx<T> (&v)[P], const cx<T>* lds, int j) {
+    constexpr int TPL = CLine<L>::TPL;
+    if constexpr (TPL % 16 == 0) {
+        const cx<T>* b = lds + pad(j);
+#pragma unroll
+        for (int s = 0; s < P; ++s) v[s] = b[padc(s * TPL)];
+    } else {
+#pragma unroll
+        for (int s = 0; s < P; ++s) v[s] = lds[pad(j + s * TPL)];
+    }
+}
+
+template <int L, typename T>
+__device__ __forceinline__ void lds_write_c(const cx<T> (&v)[P], cx<T>* lds, int j) {
+    constexpr int TPL = CLine<L>::TPL;
+    if constexpr (TPL % 16 == 0) {
+        cx<T>* b = lds + pad(j);
+#pragma unroll
+        for (int s = 0; s < P; ++s) b[padc(s * TPL)] = v[s];
+    } else {
+#pragma unroll
+        for (int s = 0; s < P; ++s) lds[pad(j + s * TPL)] = v[s];
+    }
+}
+
+template <int DIR, int L, int PASS, typename T>
+__device__ __forceinline__ void cpass(cx<T> (&v)[P], int j, const cx<T>* __restrict__ tw,
+                                      cx<T>* lds) {
+    using G = CLine<L>;
+    constexpr int R = G::radix(PASS);
+    constexpr int Ns = G::stride(PASS);
+    constexpr bool last = (PASS == G::passes - 1);
+    constexpr int Q = P / R;
+    constexpr int TPL = G::TPL;
+    constexpr int tstride = L / (Ns * R);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = j + q * TPL;
+        const int k = b & (Ns - 1);
+        if constexpr (Ns > 1) {
+            cx<T> w[R];
+            w[1] = tw[k * tstride];
+            if constexpr (R > 2) w[2] = tw[2 * k * tstride];
+            if constexpr (R > 4) w[4] = tw[4 * k * tstride];
+            if constexpr (R > 8) w[8] = tw[8 * k * tstride];
+            if constexpr (R > 2) w[3] = cmul(w[1], w[2]);
+            if constexpr (R > 4) {
+                w[5] = cmul(w[1], w[4]);
+                w[6] = cmul(w[2], w[4]);
+                w[7] = cmul(w[3], w[4]);
+            }
+            if constexpr (R > 8) {
+#pragma unroll
+                for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], w[8]);
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                v[q + r * Q] = DIR < 0 ? cmul(v[q + r * Q], w[r]) : cmulc(v[q + r * Q], w[r]);
+        }
+        if constexpr (R == 16) {
+            dft16<DIR>(v);
+        } else if constexpr (R == 8) {
+            dft8<DIR>(v[q], v[q + Q], v[q + 2 * Q], v[q + 3 * Q], v[q + 4 * Q], v[q + 5 * Q],
+                      v[q + 6 * Q], v[q + 7 * Q]);
+        } else if constexpr (R == 4) {
+            dft4<DIR>(v[q], v[q + Q], v[q + 2 * Q], v[q + 3 * Q]);
+        } else if constexpr (R == 2) {
+            dft2<DIR>(v[q], v[q + Q]);
+        }
+        if constexpr (!last) {
+            const int base = (b - k) * R + k;
+            if constexpr (Ns % 16 == 0) {
+                cx<T>* o = lds + pad(base);
+#pragma unroll
+                for (int r = 0; r < R; ++r) o[r * (Ns + Ns / 16)] = v[q + r * Q];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[pad(base + r * Ns)] = v[q + r * Q];
+            }
+        }
+    }
+}
+
+template <int DIR, int L, int PASS, typename T>
+__device__ __forceinline__ void cpasses(cx<T> (&v)[P], cx<T>* lds, int j,
+                                        const cx<T>* __restrict__ tw) {
+    if constexpr (PASS < CLine<L>::passes) {
+        if constexpr (PASS > 0) lds_read_c<L>(v, lds, j);
+        __syncthreads();
+        cpass<DIR, L, PASS>(v, j, tw, lds);
+        if constexpr (PASS + 1 < CLine<L>::passes) __syncthreads();
+        cpasses<DIR, L, PASS + 1>(v, lds, j, tw);
+    }
+}
+
+// Compile-time-length version of line_fft (same contract).
+template <int DIR, int L, typename T>
+__device__ __forceinline__ void line_fft_c(cx<T> (&v)[P], cx<T>* lds, int j,
+                                           const cx<T>* __restrict__ tw) {
+    cpasses<DIR, L, 0>(v, lds, j, tw);
+}
+
 // XCD-aware block order: MI355X dispatches workgroup b to XCD b % 8; this
 // gives each XCD a contiguous range of logical tiles so that neighbouring
 // tiles (which share cache lines of the transposed intermediates) meet in

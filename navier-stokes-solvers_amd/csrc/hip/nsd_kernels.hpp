@@ -189,6 +189,14 @@ __device__ __forceinline__ void st_ref(double* p, size_t i, cx<T> v) {
     p[2 * i + 1] = double(v.y);
 }
 
+// line transform of compile-time length LC (> 0) or runtime length g.L
+template <int DIR, int LC, typename T>
+__device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
+                                        const cx<T>* __restrict__ tw) {
+    if constexpr (LC > 0) line_fft_c<DIR, LC>(v, lds, j, tw);
+    else line_fft<DIR>(v, lds, j, g, tw);
+}
+
 // ------------------------------------------------------------ column side
 
 enum ColMode { COL_STAGE = 0, COL_PRIME = 1, COL_FWD = 2, COL_INV = 3 };
@@ -253,15 +261,17 @@ __device__ __forceinline__ cx<T> apply_op(int op, cx<T> a, cx<T> b, T kx, T ky, 
 template <typename T>
 constexpr int min_waves() { return sizeof(T) == 4 ? 4 : 2; }
 
-template <typename T, int NT, int MODE, int SCHEME>
+template <typename T, int NT, int MODE, int SCHEME, int LC>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     const Geo& G = a.geo;
-    const Line g = make_line(G.ny);
-    const int lines = NT / g.TPL;
-    const int m = threadIdx.x / g.TPL, j = threadIdx.x - m * g.TPL;
-    cx<T>* lds = smem + m * (g.L + g.L / 16);
+    const Line g = make_line(LC > 0 ? LC : G.ny);
+    const int TPL = LC > 0 ? LC / P : g.TPL;
+    const int Ll = LC > 0 ? LC : g.L;
+    const int lines = NT / TPL;
+    const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
+    cx<T>* lds = smem + m * (Ll + Ll / 16);
     const int kxi = blockIdx.x * lines + m;
     const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
     const bool active = kxi < ncols;
@@ -276,8 +286,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     // ---- forward y-transform of the mixed nonlinear field
     if constexpr (MODE == COL_STAGE || MODE == COL_FWD) {
 #pragma unroll
-        for (int s = 0; s < P; ++s) v[s] = has_data ? a.nl[mbase + j + s * g.TPL] : zero;
-        line_fft<-1>(v, lds, j, g, a.tw);
+        for (int s = 0; s < P; ++s) v[s] = has_data ? a.nl[mbase + j + s * TPL] : zero;
+        fft_any<-1, LC>(v, lds, j, g, a.tw);
     }
 
     if constexpr (MODE == COL_FWD) {
@@ -285,7 +295,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         const T norm = T(a.fwd_scale / ((double)G.nx * (double)G.ny));
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            const int jy = j + s * g.TPL;
+            const int jy = j + s * TPL;
             cx<T> r = (has_data && kept(G, kxi, jy)) ? norm * v[s] : zero;
             if (active) st_ref(a.out_ref, (size_t)jy * G.nkx + kxi, r);
         }
@@ -313,7 +323,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
 
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-        const int jy = j + s * g.TPL;
+        const int jy = j + s * TPL;
         w[s] = zero;
         if (!(has_data && kept(G, kxi, jy))) continue;
         const size_t ci = (size_t)kxi * G.nky_keep + compact_y(G, jy);
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         for (int f = 0; f < a.nout; ++f) {
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                const int jy = j + s * g.TPL;
+                const int jy = j + s * TPL;
                 const T ky = T(band_of(G, jy)) * kys;
                 const T k2 = kx * kx + ky * ky;
                 cx<T> b = zero;
@@ -408,11 +418,11 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                     b = ld_ref<T>(a.in1_ref, (size_t)jy * G.nkx + kxi);
                 v[s] = apply_op(a.ops[f], w[s], b, kx, ky, k2);
             }
-            line_fft<+1>(v, lds, j, g, a.tw);
+            fft_any<+1, LC>(v, lds, j, g, a.tw);
             if (has_data) {
                 cx<T>* out = a.mixed[f];
 #pragma unroll
-                for (int s = 0; s < P; ++s) out[mbase + j + s * g.TPL] = v[s];
+                for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
     } else {
@@ -423,21 +433,21 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             if (f == 1) {
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
-                    const T ky = T(band_of(G, j + s * g.TPL)) * kys;
+                    const T ky = T(band_of(G, j + s * TPL)) * kys;
                     const T k2 = kx * kx + ky * ky;
                     w[s] = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
                 }
             }
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                const T ky = T(band_of(G, j + s * g.TPL)) * kys;
+                const T ky = T(band_of(G, j + s * TPL)) * kys;
                 v[s] = mul_i(w[s], (f & 1) ? ky : kx);
             }
-            line_fft<+1>(v, lds, j, g, a.tw);
+            fft_any<+1, LC>(v, lds, j, g, a.tw);
             if (has_data) {
                 cx<T>* out = a.mixed[f];
 #pragma unroll
-                for (int s = 0; s < P; ++s) out[mbase + j + s * g.TPL] = v[s];
+                for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
     }
@@ -465,15 +475,17 @@ struct RowArgs {
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
 // the kept mixed columns (c2r convention: imaginary parts of the kx = 0
 // values are ignored, as FFTW's c2r does), then read the standard pattern.
-template <typename T>
+template <int LC, typename T>
 __device__ __forceinline__ void build_row_pair(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                                const Geo& G, const cx<T>* A, const cx<T>* B,
                                                int y, bool active) {
+    const int TPL = LC > 0 ? LC / P : g.TPL;
+    const int Ll = LC > 0 ? LC : g.L;
     const cx<T> zero = mk(T(0), T(0));
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-        const int k = j + s * g.TPL;
+        const int k = j + s * TPL;
         if (k <= G.kxmax) {
             cx<T> a = zero, b = zero;
             if (active) {
@@ -485,27 +497,30 @@ __device__ __forceinline__ void build_row_pair(cx<T> (&v)[P], cx<T>* lds, int j,
                 b.y = T(0);
             }
             lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
-            if (k > 0) lds[pad(g.L - k)] = mk(a.x + b.y, b.x - a.y);
-        } else if (k < g.L - G.kxmax) {
+            if (k > 0) lds[pad(Ll - k)] = mk(a.x + b.y, b.x - a.y);
+        } else if (k < Ll - G.kxmax) {
             lds[pad(k)] = zero;
         }
     }
     __syncthreads();
-    lds_read_std(v, lds, j, g.TPL);
+    if constexpr (LC > 0) lds_read_c<LC>(v, lds, j);
+    else lds_read_std(v, lds, j, TPL);
 }
 
 // Same Hermitian extension for Z = F(., y) + i F(., y+1): the two rows of one
 // field, fetched with a single 2-element load per column (the rows are
 // adjacent in the [kx][y] mixed layout).
-template <typename T>
+template <int LC, typename T>
 __device__ __forceinline__ void build_two_rows(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                                const Geo& G, const cx<T>* F, int y,
                                                bool active) {
+    const int TPL = LC > 0 ? LC / P : g.TPL;
+    const int Ll = LC > 0 ? LC : g.L;
     const cx<T> zero = mk(T(0), T(0));
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-        const int k = j + s * g.TPL;
+        const int k = j + s * TPL;
         if (k <= G.kxmax) {
             cx<T> a = zero, b = zero;
             if (active) {
@@ -518,31 +533,35 @@ __device__ __forceinline__ void build_two_rows(cx<T> (&v)[P], cx<T>* lds, int j,
                 b.y = T(0);
             }
             lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
-            if (k > 0) lds[pad(g.L - k)] = mk(a.x + b.y, b.x - a.y);
-        } else if (k < g.L - G.kxmax) {
+            if (k > 0) lds[pad(Ll - k)] = mk(a.x + b.y, b.x - a.y);
+        } else if (k < Ll - G.kxmax) {
             lds[pad(k)] = zero;
         }
     }
     __syncthreads();
-    lds_read_std(v, lds, j, g.TPL);
+    if constexpr (LC > 0) lds_read_c<LC>(v, lds, j);
+    else lds_read_std(v, lds, j, TPL);
 }
 
 // Forward x-transform of two real rows packed as v = ra + i rb, then
 // separation of the two half spectra and store of the kept columns into
 // the mixed layout at rows y and y+1.
-template <typename T>
+template <int LC, typename T>
 __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                                const Geo& G, const cx<T>* tw, cx<T>* out, int y,
                                                bool active) {
-    line_fft<-1>(v, lds, j, g, tw);
-    lds_write_std(v, lds, j, g.TPL);
+    const int TPL = LC > 0 ? LC / P : g.TPL;
+    const int Ll = LC > 0 ? LC : g.L;
+    fft_any<-1, LC>(v, lds, j, g, tw);
+    if constexpr (LC > 0) lds_write_c<LC>(v, lds, j);
+    else lds_write_std(v, lds, j, TPL);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-        const int k = j + s * g.TPL;
+        const int k = j + s * TPL;
         if (k <= G.kxmax && active) {
             const cx<T> zk = lds[pad(k)];
-            const cx<T> zm = lds[pad((g.L - k) & (g.L - 1))];
+            const cx<T> zm = lds[pad((Ll - k) & (Ll - 1))];
             const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
             const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
             out[(size_t)k * G.ny + y] = ja;
@@ -551,16 +570,18 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
     }
 }
 
-template <typename T, int NT, int MODE>
+template <typename T, int NT, int MODE, int LC>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     __shared__ double red[NT / 64];
     const Geo& G = a.geo;
-    const Line g = make_line(G.nx);
-    const int lines = NT / g.TPL;
-    const int m = threadIdx.x / g.TPL, j = threadIdx.x - m * g.TPL;
-    cx<T>* lds = smem + m * (g.L + g.L / 16);
+    const Line g = make_line(LC > 0 ? LC : G.nx);
+    const int TPL = LC > 0 ? LC / P : g.TPL;
+    const int Ll = LC > 0 ? LC : g.L;
+    const int lines = NT / TPL;
+    const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
+    cx<T>* lds = smem + m * (Ll + Ll / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     cx<T> v[P];
@@ -570,12 +591,12 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         const bool active = y < G.ny;
         for (int f = 0; f < a.nfields; f += 2) {
             const cx<T>* B = (f + 1 < a.nfields) ? a.mixed[f + 1] : nullptr;
-            build_row_pair(v, lds, j, g, G, a.mixed[f], B, y, active);
-            line_fft<+1>(v, lds, j, g, a.tw);
+            build_row_pair<LC>(v, lds, j, g, G, a.mixed[f], B, y, active);
+            fft_any<+1, LC>(v, lds, j, g, a.tw);
             if (active) {
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
-                    const size_t o = (size_t)y * G.nx + j + s * g.TPL;
+                    const size_t o = (size_t)y * G.nx + j + s * TPL;
                     if (a.out_ref[f]) a.out_ref[f][o] = double(v[s].x);
                     else a.out[f][o] = v[s].x;
                     if (B) {
@@ -594,12 +615,12 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     if constexpr (MODE == ROW_R2C) {
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            const int x = j + s * g.TPL;
+            const int x = j + s * TPL;
             v[s] = active ? mk(T(a.in_ref[(size_t)y0 * G.nx + x]),
                                T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]))
                           : mk(T(0), T(0));
         }
-        fwd_rows_store(v, lds, j, g, G, a.tw, a.nl, y0, active);
+        fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
         return;
     }
 
@@ -608,35 +629,35 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // in the order psi_x, psi_y, omega_x, omega_y, so that only two packed
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
-    double u2max = 0.0;
-    build_two_rows(v, lds, j, g, G, a.mixed[0], y0, active);  // psi_x
-    line_fft<+1>(v, lds, j, g, a.tw);
+    T u2max = T(0);
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[0], y0, active);  // psi_x
+    fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) X[s] = v[s];
-    build_two_rows(v, lds, j, g, G, a.mixed[1], y0, active);  // psi_y
-    line_fft<+1>(v, lds, j, g, a.tw);
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[1], y0, active);  // psi_y
+    fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) {
         // CFL speed: u = psi_y, v = -psi_x (spectral_core.c:30-31)
-        double sa = (double)X[s].x * X[s].x + (double)v[s].x * v[s].x;
-        double sb = (double)X[s].y * X[s].y + (double)v[s].y * v[s].y;
-        if (sa != sa) sa = __builtin_inf();  // NaN must win the max: divergence check
-        if (sb != sb) sb = __builtin_inf();
+        T sa = X[s].x * X[s].x + v[s].x * v[s].x;
+        T sb = X[s].y * X[s].y + v[s].y * v[s].y;
+        if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max: divergence check
+        if (sb != sb) sb = T(__builtin_inf());
         u2max = sa > u2max ? sa : u2max;
         u2max = sb > u2max ? sb : u2max;
         Jp[s] = v[s];  // psi_y, folded with omega_x next
     }
-    build_two_rows(v, lds, j, g, G, a.mixed[2], y0, active);  // omega_x
-    line_fft<+1>(v, lds, j, g, a.tw);
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[2], y0, active);  // omega_x
+    fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
-    build_two_rows(v, lds, j, g, G, a.mixed[3], y0, active);  // omega_y
-    line_fft<+1>(v, lds, j, g, a.tw);
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[3], y0, active);  // omega_y
+    fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s)
         v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
-    fwd_rows_store(v, lds, j, g, G, a.tw, a.nl, y0, active);
-    if (a.track_umax) block_max_to(active ? u2max : 0.0, &a.sc->umax2_bits, red);
+    fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
+    if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc->umax2_bits, red);
 }
 
 }  // namespace nsd
