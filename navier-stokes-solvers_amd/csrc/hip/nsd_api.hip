@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/ns_device.h"
+#include "ns_device.h"
 #include "nsd_stage.hpp"
 
 using namespace nsd;

@@ -192,7 +192,7 @@ __host__ __device__ inline Line make_line(int L) {
 }
 
 // threads per workgroup for lines of length L (a workgroup holds whole lines)
-__host__ __device__ inline int threads_for(int L) { return L / P > 256 ? L / P : 256; }
+__host__ __device__ constexpr int threads_for(int L) { return L / P > 256 ? L / P : 256; }
 
 // One Stockham pass of radix R on the thread's P registers (Q = P/R
 // butterflies per thread).  If `last`, the outputs stay in v (standard
@@ -413,6 +413,103 @@ template <int DIR, int L, typename T>
 __device__ __forceinline__ void line_fft_c(cx<T> (&v)[P], cx<T>* lds, int j,
                                            const cx<T>* __restrict__ tw) {
     cpasses<DIR, L, 0>(v, lds, j, tw);
+}
+
+// Two independent transforms of compile-time length L run in lock-step: one
+// barrier pair per pass serves both, the twiddles are computed once, and the
+// two dependency chains interleave.  lds1/lds2 are two separate line regions.
+template <int DIR, int L, int PASS, typename T>
+__device__ __forceinline__ void cpass2(cx<T> (&v1)[P], cx<T> (&v2)[P], int j,
+                                       const cx<T>* __restrict__ tw, cx<T>* lds1, cx<T>* lds2) {
+    using G = CLine<L>;
+    constexpr int R = G::radix(PASS);
+    constexpr int Ns = G::stride(PASS);
+    constexpr bool last = (PASS == G::passes - 1);
+    constexpr int Q = P / R;
+    constexpr int TPL = G::TPL;
+    constexpr int tstride = L / (Ns * R);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = j + q * TPL;
+        const int k = b & (Ns - 1);
+        if constexpr (Ns > 1) {
+            cx<T> w[R];
+            w[1] = tw[k * tstride];
+            if constexpr (R > 2) w[2] = tw[2 * k * tstride];
+            if constexpr (R > 4) w[4] = tw[4 * k * tstride];
+            if constexpr (R > 8) w[8] = tw[8 * k * tstride];
+            if constexpr (R > 2) w[3] = cmul(w[1], w[2]);
+            if constexpr (R > 4) {
+                w[5] = cmul(w[1], w[4]);
+                w[6] = cmul(w[2], w[4]);
+                w[7] = cmul(w[3], w[4]);
+            }
+            if constexpr (R > 8) {
+#pragma unroll
+                for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], w[8]);
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                v1[q + r * Q] = DIR < 0 ? cmul(v1[q + r * Q], w[r]) : cmulc(v1[q + r * Q], w[r]);
+                v2[q + r * Q] = DIR < 0 ? cmul(v2[q + r * Q], w[r]) : cmulc(v2[q + r * Q], w[r]);
+            }
+        }
+        if constexpr (R == 16) {
+            dft16<DIR>(v1);
+            dft16<DIR>(v2);
+        } else if constexpr (R == 8) {
+            dft8<DIR>(v1[q], v1[q + Q], v1[q + 2 * Q], v1[q + 3 * Q], v1[q + 4 * Q], v1[q + 5 * Q],
+                      v1[q + 6 * Q], v1[q + 7 * Q]);
+            dft8<DIR>(v2[q], v2[q + Q], v2[q + 2 * Q], v2[q + 3 * Q], v2[q + 4 * Q], v2[q + 5 * Q],
+                      v2[q + 6 * Q], v2[q + 7 * Q]);
+        } else if constexpr (R == 4) {
+            dft4<DIR>(v1[q], v1[q + Q], v1[q + 2 * Q], v1[q + 3 * Q]);
+            dft4<DIR>(v2[q], v2[q + Q], v2[q + 2 * Q], v2[q + 3 * Q]);
+        } else if constexpr (R == 2) {
+            dft2<DIR>(v1[q], v1[q + Q]);
+            dft2<DIR>(v2[q], v2[q + Q]);
+        }
+        if constexpr (!last) {
+            const int base = (b - k) * R + k;
+            if constexpr (Ns % 16 == 0) {
+                cx<T>* o1 = lds1 + pad(base);
+                cx<T>* o2 = lds2 + pad(base);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    o1[r * (Ns + Ns / 16)] = v1[q + r * Q];
+                    o2[r * (Ns + Ns / 16)] = v2[q + r * Q];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    lds1[pad(base + r * Ns)] = v1[q + r * Q];
+                    lds2[pad(base + r * Ns)] = v2[q + r * Q];
+                }
+            }
+        }
+    }
+}
+
+template <int DIR, int L, int PASS, typename T>
+__device__ __forceinline__ void cpasses2(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* lds1,
+                                         cx<T>* lds2, int j, const cx<T>* __restrict__ tw) {
+    if constexpr (PASS < CLine<L>::passes) {
+        if constexpr (PASS > 0) {
+            lds_read_c<L>(v1, lds1, j);
+            lds_read_c<L>(v2, lds2, j);
+        }
+        __syncthreads();
+        cpass2<DIR, L, PASS>(v1, v2, j, tw, lds1, lds2);
+        if constexpr (PASS + 1 < CLine<L>::passes) __syncthreads();
+        cpasses2<DIR, L, PASS + 1>(v1, v2, lds1, lds2, j, tw);
+    }
+}
+
+// Two transforms in lock-step (same contract as line_fft_c for each region).
+template <int DIR, int L, typename T>
+__device__ __forceinline__ void line_fft2_c(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* lds1,
+                                            cx<T>* lds2, int j, const cx<T>* __restrict__ tw) {
+    cpasses2<DIR, L, 0>(v1, v2, lds1, lds2, j, tw);
 }
 
 // XCD-aware block order: MI355X dispatches workgroup b to XCD b % 8; this

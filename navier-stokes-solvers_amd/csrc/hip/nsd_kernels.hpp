@@ -197,6 +197,39 @@ __device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const 
     else line_fft<DIR>(v, lds, j, g, tw);
 }
 
+// Kept bands of the 2/3 rule (spectral_initialization.c:46-48) for lines of
+// compile-time length LC: last kept kx of a row of length LC and last kept
+// |ky| of a column of length LC.  slot_kind classifies register slot s of
+// the standard pattern (indices s*TPL .. s*TPL+TPL-1): 1 = always in the
+// positive kept band, 2 = always in the negative (mirrored) kept band,
+// 0 = never kept, 3 = straddles a band edge (decided per lane).
+template <int LC>
+struct Band {
+    static constexpr int kxm = (2 * (LC / 2 + 1)) / 3;
+    static constexpr int kym = (2 * LC) / 6;
+};
+
+// Whether the time-step kernels run their inverse transforms two at a time
+// (needs two line regions per line in LDS; kept within 80 KB per workgroup
+// so that two workgroups share a CU).
+template <typename T, int LC>
+constexpr bool paired_lines() {
+    return LC > 0 && 2 * (LC + LC / 16) * (int)sizeof(cx<T>) * (threads_for(LC) * P / LC) <=
+                         80 * 1024;
+}
+template <typename T, int LC>
+constexpr int line_regions() {
+    return paired_lines<T, LC>() ? 2 : 1;
+}
+
+constexpr int slot_kind(int s, int L, int km) {
+    const int tpl = L / P, lo = s * tpl, hi = lo + tpl - 1;
+    if (hi <= km) return 1;
+    if (lo >= L - km && lo > 0) return 2;
+    if (lo > km && hi < L - km) return 0;
+    return 3;
+}
+
 // ------------------------------------------------------------ column side
 
 enum ColMode { COL_STAGE = 0, COL_PRIME = 1, COL_FWD = 2, COL_INV = 3 };
@@ -271,7 +304,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    cx<T>* lds = smem + m * (Ll + Ll / 16);
+    constexpr int NB = line_regions<T, LC>();
+    cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
+    cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int kxi = blockIdx.x * lines + m;
     const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
     const bool active = kxi < ncols;
@@ -321,13 +356,19 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const double sfd_alpha = (a.phys.sfd_tau > 0.0) ? fmin(fmax(dt / a.phys.sfd_tau, 0.0), 1.0) : 0.0;
     const T sqh = T(sqrt(dt));
 
+    // y-band of the 2/3 mask: compile-time for compile-time column lengths
+    const int kym = LC > 0 ? Band<LC>::kym : G.kymax;
+    const int ncy = LC > 0 ? 2 * Band<LC>::kym + 1 : G.nky_keep;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
+        const int kind = LC > 0 ? slot_kind(s, LC, Band<LC>::kym) : 3;
         const int jy = j + s * TPL;
         w[s] = zero;
-        if (!(has_data && kept(G, kxi, jy))) continue;
-        const size_t ci = (size_t)kxi * G.nky_keep + compact_y(G, jy);
-        const T ky = T(band_of(G, jy)) * T(2.0 * kPi / G.Ly);
+        if (kind == 0 || !has_data) continue;
+        const bool pos = kind == 1 || (kind == 3 && jy <= kym);
+        if (kind == 3 && !pos && jy < Ll - kym) continue;
+        const size_t ci = (size_t)kxi * ncy + (pos ? jy : jy - (Ll - ncy));
+        const T ky = T(pos ? jy : jy - Ll) * T(2.0 * kPi / G.Ly);
         if constexpr (MODE == COL_PRIME) {
             w[s] = a.state[ci];
             continue;
@@ -425,6 +466,38 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
+    } else if constexpr (paired_lines<T, LC>()) {
+        // outputs 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99),
+        // transformed two at a time: (omega_y, omega_x), then w becomes
+        // psi_hat = -w/k^2 in place and (psi_y, psi_x)
+        cx<T> v2[P];
+#pragma unroll 1
+        for (int f = 3; f >= 1; f -= 2) {
+            if (f == 1) {
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    const T ky = T(band_of(G, j + s * TPL)) * kys;
+                    const T k2 = kx * kx + ky * ky;
+                    w[s] = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                v[s] = mul_i(w[s], ky);
+                v2[s] = mul_i(w[s], kx);
+            }
+            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);
+            if (has_data) {
+                cx<T>* o1 = a.mixed[f];
+                cx<T>* o2 = a.mixed[f - 1];
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    o1[mbase + j + s * TPL] = v[s];
+                    o2[mbase + j + s * TPL] = v2[s];
+                }
+            }
+        }
     } else {
         // outputs 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99);
         // the omega-based pair first, then w becomes psi_hat = -w/k^2 in place
@@ -511,17 +584,22 @@ __device__ __forceinline__ void build_row_pair(cx<T> (&v)[P], cx<T>* lds, int j,
 // field, fetched with a single 2-element load per column (the rows are
 // adjacent in the [kx][y] mixed layout).
 template <int LC, typename T>
-__device__ __forceinline__ void build_two_rows(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
-                                               const Geo& G, const cx<T>* F, int y,
-                                               bool active) {
+__device__ __forceinline__ void stage_two_rows(cx<T>* lds, int j, const Line& g, const Geo& G,
+                                               const cx<T>* F, int y, bool active) {
     const int TPL = LC > 0 ? LC / P : g.TPL;
     const int Ll = LC > 0 ? LC : g.L;
+    const int kxm = LC > 0 ? Band<LC>::kxm : G.kxmax;
     const cx<T> zero = mk(T(0), T(0));
-    __syncthreads();
 #pragma unroll
     for (int s = 0; s < P; ++s) {
+        const int kind = LC > 0 ? slot_kind(s, LC, Band<LC>::kxm) : 3;
         const int k = j + s * TPL;
-        if (k <= G.kxmax) {
+        if (kind == 2) continue;  // written by the owner of Ll - k
+        if (kind == 0) {
+            lds[pad(k)] = zero;
+            continue;
+        }
+        if (kind == 1 || k <= kxm) {
             cx<T> a = zero, b = zero;
             if (active) {
                 const cx<T>* src = F + (size_t)k * G.ny + y;
@@ -534,13 +612,35 @@ __device__ __forceinline__ void build_two_rows(cx<T> (&v)[P], cx<T>* lds, int j,
             }
             lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
             if (k > 0) lds[pad(Ll - k)] = mk(a.x + b.y, b.x - a.y);
-        } else if (k < Ll - G.kxmax) {
+        } else if (k < Ll - kxm) {
             lds[pad(k)] = zero;
         }
     }
+}
+
+template <int LC, typename T>
+__device__ __forceinline__ void build_two_rows(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
+                                               const Geo& G, const cx<T>* F, int y,
+                                               bool active) {
+    __syncthreads();
+    stage_two_rows<LC>(lds, j, g, G, F, y, active);
     __syncthreads();
     if constexpr (LC > 0) lds_read_c<LC>(v, lds, j);
-    else lds_read_std(v, lds, j, TPL);
+    else lds_read_std(v, lds, j, g.TPL);
+}
+
+// both fields F1, F2 for rows y, y+1 into two line regions
+template <int LC, typename T>
+__device__ __forceinline__ void build_two_rows2(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* lds1,
+                                                cx<T>* lds2, int j, const Line& g, const Geo& G,
+                                                const cx<T>* F1, const cx<T>* F2, int y,
+                                                bool active) {
+    __syncthreads();
+    stage_two_rows<LC>(lds1, j, g, G, F1, y, active);
+    stage_two_rows<LC>(lds2, j, g, G, F2, y, active);
+    __syncthreads();
+    lds_read_c<LC>(v1, lds1, j);
+    lds_read_c<LC>(v2, lds2, j);
 }
 
 // Forward x-transform of two real rows packed as v = ra + i rb, then
@@ -552,14 +652,17 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
                                                bool active) {
     const int TPL = LC > 0 ? LC / P : g.TPL;
     const int Ll = LC > 0 ? LC : g.L;
+    const int kxm = LC > 0 ? Band<LC>::kxm : G.kxmax;
     fft_any<-1, LC>(v, lds, j, g, tw);
     if constexpr (LC > 0) lds_write_c<LC>(v, lds, j);
     else lds_write_std(v, lds, j, TPL);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < P; ++s) {
+        const int kind = LC > 0 ? slot_kind(s, LC, Band<LC>::kxm) : 3;
+        if (kind == 0 || kind == 2) continue;
         const int k = j + s * TPL;
-        if (k <= G.kxmax && active) {
+        if ((kind == 1 || k <= kxm) && active) {
             const cx<T> zk = lds[pad(k)];
             const cx<T> zm = lds[pad((Ll - k) & (Ll - 1))];
             const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
@@ -581,7 +684,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    cx<T>* lds = smem + m * (Ll + Ll / 16);
+    constexpr int NB = line_regions<T, LC>();
+    cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
+    cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     cx<T> v[P];
@@ -630,6 +735,30 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
     T u2max = T(0);
+    if constexpr (paired_lines<T, LC>()) {
+        cx<T> v2[P];
+        build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0], a.mixed[1], y0, active);
+        line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            T sa = v[s].x * v[s].x + v2[s].x * v2[s].x;  // u = psi_y, v = -psi_x
+            T sb = v[s].y * v[s].y + v2[s].y * v2[s].y;
+            if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max
+            if (sb != sb) sb = T(__builtin_inf());
+            u2max = sa > u2max ? sa : u2max;
+            u2max = sb > u2max ? sb : u2max;
+            X[s] = v[s];
+            Jp[s] = v2[s];
+        }
+        build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[2], a.mixed[3], y0, active);
+        line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // omega_x, omega_y
+#pragma unroll
+        for (int s = 0; s < P; ++s)
+            v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
+        fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
+        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc->umax2_bits, red);
+        return;
+    }
     build_two_rows<LC>(v, lds, j, g, G, a.mixed[0], y0, active);  // psi_x
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll

@@ -22,7 +22,7 @@ void row_stage_L(const RowArgs<T>& a, int items, hipStream_t st) {
     constexpr int NT = nt_for<L>();
     constexpr int lines = NT * P / L;
     const int blocks = (items + lines - 1) / lines;
-    const size_t lds = (size_t)lines * (L + L / 16) * sizeof(cx<T>);
+    const size_t lds = (size_t)lines * line_regions<T, L>() * (L + L / 16) * sizeof(cx<T>);
     hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L>), dim3(blocks), dim3(NT), lds, st, a);
 }
 
@@ -32,7 +32,7 @@ void col_fused_L(const ColArgs<T>& a, int ncols, hipStream_t st) {
     constexpr int NT = nt_for<L>();
     constexpr int lines = NT * P / L;
     const int blocks = (ncols + lines - 1) / lines;
-    const size_t lds = (size_t)lines * (L + L / 16) * sizeof(cx<T>);
+    const size_t lds = (size_t)lines * line_regions<T, L>() * (L + L / 16) * sizeof(cx<T>);
     if (a.mode == COL_PRIME)
         hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L>), dim3(blocks), dim3(NT), lds, st, a);
     else if (a.phys.scheme == 0)
