@@ -119,6 +119,17 @@ class AdaptiveTimeParams(C.Structure):
                 ("safety_factor", C.c_double), ("adapt_enabled", C.c_int)]
 
 
+class NsdParams(C.Structure):  # ns_device.h
+    _fields_ = [("Re", C.c_double), ("drag", C.c_double), ("cfl_factor", C.c_double),
+                ("scheme", C.c_int), ("hv_on", C.c_int), ("hv_order", C.c_int),
+                ("hv_coeff", C.c_double), ("hv_kc", C.c_double), ("hv_invbw", C.c_double),
+                ("sfd_on", C.c_int), ("sfd_strength", C.c_double), ("sfd_tau", C.c_double),
+                ("force_on", C.c_int), ("force_colored", C.c_int), ("force_kmin", C.c_double),
+                ("force_kmax", C.c_double), ("force_energy_target", C.c_double),
+                ("force_tau", C.c_double), ("force_rate", C.c_double),
+                ("force_seed", C.c_ulonglong)]
+
+
 _lib = None
 
 
@@ -220,6 +231,47 @@ def load():
     L.nsd_bandwidth_probe.restype = C.c_int
     L.nsd_synchronize.argtypes = [C.c_void_p]
     L.nsd_synchronize.restype = C.c_int
+    V, D, I = C.c_void_p, c_double_p, c_int_p
+    ull = C.c_ulonglong
+    dev = {
+        "nsd_create": ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_char_p,
+                        C.c_size_t], V),
+        "nsd_destroy": ([V], None),
+        "nsd_error": ([V], C.c_char_p),
+        "nsd_set_params": ([V, C.POINTER(NsdParams)], C.c_int),
+        "nsd_precision": ([V], C.c_int),
+        "nsd_state_upload": ([V, D], C.c_int),
+        "nsd_state_download": ([V, D], C.c_int),
+        "nsd_sfd_upload": ([V, D], C.c_int),
+        "nsd_sfd_download": ([V, D], C.c_int),
+        "nsd_state_random_phase": ([V, ull, C.c_double, C.c_double], C.c_int),
+        "nsd_set_time": ([V, C.c_double, C.c_double], C.c_int),
+        "nsd_get_time": ([V, D, D, I], C.c_int),
+        "nsd_set_forcing_counter": ([V, ull], C.c_int),
+        "nsd_get_forcing_counter": ([V, C.POINTER(ull)], C.c_int),
+        "nsd_state_derived": ([V, D, D, D, D, D, D], C.c_int),
+        "nsd_state_energy": ([V, D], C.c_int),
+        "nsd_grids": ([V, D, D, D, D, D, I], C.c_int),
+        "nsd_hv_weights": ([V, C.c_int, C.c_double, C.c_int, C.c_double, D, D, D, D], C.c_int),
+        "nsd_physical_to_spectral": ([V, D, D], C.c_int),
+        "nsd_spectral_to_physical": ([V, D, D], C.c_int),
+        "nsd_dealias": ([V, D], C.c_int),
+        "nsd_velocity": ([V, D, D, D, D], C.c_int),
+        "nsd_derivatives": ([V, D, D, D, D, D, D, D], C.c_int),
+        "nsd_nonlinear": ([V, D, D, D], C.c_int),
+        "nsd_hv_apply": ([V, D, D, D], C.c_int),
+        "nsd_hv_post_step": ([V, D, D, C.c_double], C.c_int),
+        "nsd_taylor_green": ([V, D, D, D, D], C.c_int),
+        "nsd_phys_reduce": ([V, D, D, D, D], C.c_int),
+        "nsd_spec_reduce": ([V, D, D], C.c_int),
+        "nsd_spectrum": ([V, D, D, D, I, C.c_int], C.c_int),
+        "nsd_forcing_generate": ([C.c_int, I, C.c_int, ull, ull, C.c_int, C.c_double,
+                                  C.c_double, C.c_double, D, D, C.c_char_p, C.c_size_t],
+                                 C.c_int),
+    }
+    for name, (args, res) in dev.items():
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = res
     _lib = L
     return L
 

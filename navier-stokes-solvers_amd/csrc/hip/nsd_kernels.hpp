@@ -290,6 +290,12 @@ __device__ __forceinline__ cx<T> apply_op(int op, cx<T> a, cx<T> b, T kx, T ky, 
     return mul_i(psi, -kx);  // OP_V0
 }
 
+// (1 + em1) * w with em1 = factor - 1, rounded once per component
+template <typename T>
+__device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
+    return mk(fma(em1, w.x, w.x), fma(em1, w.y, w.y));
+}
+
 // occupancy floor handed to the register allocator: waves per SIMD
 template <typename T>
 constexpr int min_waves() { return sizeof(T) == 4 ? 4 : 2; }
@@ -409,19 +415,22 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
             if (a.stage < 3) a.stagebuf[ci] = wn;
         } else {
-            // integrating factor, L = -(k^2/Re + w_hv + mu)
+            // integrating factor, L = -(k^2/Re + w_hv + mu).  The factors are
+            // carried as E - 1 (expm1): near k = 0 the per-step decay L h is
+            // ~1e-5, which 1 + (E - 1) keeps to full relative precision while
+            // E itself, rounded to FP32, would lose ~0.3 % of it every step.
             const T L = -(k2 * T(a.phys.inv_Re) + T(a.phys.drag) + hv_weight_t<T>(a.phys, kmag));
-            const T E2 = exp(L * (T(0.5) * h));
-            const T E = E2 * E2;
+            const T e2 = expm1(L * (T(0.5) * h));  // E2 - 1, E2 = exp(L h / 2)
+            const T e1 = e2 * (T(2) + e2);          // E - 1,  E = E2^2
             if (a.stage == 0) {
-                a.acc[ci] = E * w0 + (h / T(6) * E) * N;
-                wn = E2 * (w0 + (T(0.5) * h) * N);
+                a.acc[ci] = grow(w0, e1) + (h / T(6)) * grow(N, e1);
+                wn = grow(w0 + (T(0.5) * h) * N, e2);
             } else if (a.stage == 1) {
-                a.acc[ci] = a.acc[ci] + (h / T(3) * E2) * N;
-                wn = E2 * w0 + (T(0.5) * h) * N;
+                a.acc[ci] = a.acc[ci] + (h / T(3)) * grow(N, e2);
+                wn = grow(w0, e2) + (T(0.5) * h) * N;
             } else if (a.stage == 2) {
-                a.acc[ci] = a.acc[ci] + (h / T(3) * E2) * N;
-                wn = E * w0 + (h * E2) * N;
+                a.acc[ci] = a.acc[ci] + (h / T(3)) * grow(N, e2);
+                wn = grow(w0, e1) + h * grow(N, e2);
             } else {
                 wn = a.acc[ci] + (h / T(6)) * N;
             }

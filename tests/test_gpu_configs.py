@@ -1,0 +1,139 @@
+"""GPU runs of the two large configurations of the hot path.
+
+1024^2 decaying turbulence (Re = 1e4, del^8 hyperviscosity as an integrating
+factor, adaptive CFL 0.5, 1000 steps) against the oracle's fixture
+tests/golden/config2_golden.npz (make_config2.py), in FP64.  Tolerances:
+time 1e-11 relative, energy and enstrophy 1e-11 relative at every record, the
+shell spectrum 1e-7 relative on every shell holding more than 1e-12 of the
+peak, the low modes 1e-8 relative L2.  Both codes evaluate the same formulas
+in FP64 with different FFT algorithms; the round-off difference grows with
+the flow's chaotic amplification over 1000 steps, which these bounds leave
+room for while any algorithmic difference (a term, a sign, a mode) fails them
+by orders of magnitude.
+
+4096^2 forced turbulence (the throughput workload) in FP32 stages against
+the same run in FP64 on the GPU: single-step agreement to FP32 round-off
+(1e-5 relative L2), then the error growth of energy, enstrophy and the field
+over 2000 steps, printed, with energy and enstrophy held to 1e-5 relative.
+The integrating factors are carried as E - 1 so that the slow decay of the
+largest scales (L dt ~ 1e-5) is not rounded away in FP32; with E rounded to
+FP32 directly, energy drifted by 6e-5 over these 2000 steps.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_l2
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2 * np.pi
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu] {request.node.name}", flush=True)
+
+
+def spectrum(ns, s, max_k):
+    L = ns.load()
+    ek = np.zeros(max_k)
+    ck = np.zeros(max_k, dtype=np.int32)
+    s.check(L.nsd_spectrum(s.ctx, ns._dp(s.spec("u_hat")), ns._dp(s.spec("v_hat")), ns._dp(ek),
+                           ck.ctypes.data_as(C.POINTER(C.c_int)), max_k))
+    m = ck > 0
+    ek[1:][m[1:]] /= ck[1:][m[1:]]
+    return ek, ck
+
+
+def test_config2_decaying_1024_vs_oracle_fixture(ns):
+    G = np.load(os.path.join(GOLDEN, "config2_golden.npz"))
+    N, Re, seed, steps, every, klow = G["meta"]
+    N, seed, steps, every, klow = int(N), int(seed), int(steps), int(every), int(klow)
+    s = ns.Spectral(N, N, TWO_PI, TWO_PI, float(Re))
+    s.init(taylor_green=False)
+    kmax = np.sqrt(2.0) * N / 2.0
+    s.hv_set(enabled=1, coefficient=1e-16 * kmax ** 8, order=8, cutoff_ratio=0.0)
+    s.set_scheme(True)
+    s.d.cfl_factor = 0.5
+    s.random_phase(seed, k_peak=10.0, energy=1.0)
+    s.sync_device()
+    hist = G["history"]
+    e0, z0 = s.device_diagnostics()
+    assert e0 == pytest.approx(hist[0, 3], rel=1e-12)
+    assert z0 == pytest.approx(hist[0, 4], rel=1e-12)
+    worst = 0.0
+    for r in range(1, hist.shape[0]):
+        assert s.advance_n(every, True) == 0
+        e, z = s.device_diagnostics()
+        step, t, dt, E, Z = hist[r]
+        errs = (abs(s.d.t / t - 1), abs(e / E - 1), abs(z / Z - 1))
+        worst = max(worst, *errs)
+        print(f"step {int(step)}: t {s.d.t:.9f} rel err t {errs[0]:.1e} E {errs[1]:.1e} "
+              f"Z {errs[2]:.1e}", flush=True)
+        assert errs[0] <= 1e-11
+        assert errs[1] <= 1e-11 and errs[2] <= 1e-11
+    assert s.sync_host() == 0
+    ek, ck = spectrum(ns, s, len(G["spectrum"]))
+    assert np.array_equal(ck, G["counts"])
+    ref = G["spectrum"]
+    sig = ref > 1e-12 * ref.max()
+    serr = np.abs(ek[sig] / ref[sig] - 1).max()
+    w = s.spec("omega_hat")
+    low = np.concatenate([w[: klow + 1, : klow + 1], w[N - klow:, : klow + 1]], axis=0)
+    lerr = rel_l2(low, G["omega_low"])
+    print(f"worst history rel err {worst:.1e}; spectrum {serr:.1e} over {sig.sum()} shells; "
+          f"low modes rel L2 {lerr:.1e}", flush=True)
+    assert serr <= 1e-7
+    assert lerr <= 1e-8
+    s.close()
+
+
+def forced_4096(ns, fp32):
+    n = 4096
+    s = ns.Spectral(n, n, TWO_PI, TWO_PI, 1e4)
+    s.init(taylor_green=False)
+    kmax = np.sqrt(2.0) * n / 2.0
+    s.hv_set(enabled=1, coefficient=1e-16 * kmax ** 8, order=8, cutoff_ratio=0.0)
+    s.set_precision(fp32)
+    s.set_scheme(True)
+    s.set_drag(0.1)
+    s.set_forcing(4.0, 6.0, 1e-3, seed=42)
+    s.d.cfl_factor = 0.5
+    s.random_phase(42, k_peak=10.0, energy=1.0)
+    s.sync_device()
+    return s
+
+
+def test_config3_forced_4096_fp32_vs_fp64(ns):
+    runs = [forced_4096(ns, False), forced_4096(ns, True)]
+    # one step: FP32 round-off only
+    for s in runs:
+        assert s.advance_n(1, True) == 0
+        assert s.sync_host() == 0
+    w64, w32 = (s.spec("omega_hat").copy() for s in runs)
+    e1 = rel_l2(w32, w64)
+    print(f"1 step: omega_hat rel L2 {e1:.2e}", flush=True)
+    assert e1 <= 1e-5
+    worst_e = worst_z = 0.0
+    for chunk in range(1, 11):
+        diag = []
+        for s in runs:
+            assert s.advance_n(200 if chunk > 1 else 199, True) == 0
+            diag.append(s.device_diagnostics())
+        (E64, Z64), (E32, Z32) = diag
+        de, dz = abs(E32 / E64 - 1), abs(Z32 / Z64 - 1)
+        worst_e, worst_z = max(worst_e, de), max(worst_z, dz)
+        print(f"step {chunk * 200}: t64 {runs[0].d.t:.6f} t32 {runs[1].d.t:.6f} "
+              f"E {E64:.6e} rel err {de:.1e}  Z {Z64:.6e} rel err {dz:.1e}", flush=True)
+    for s in runs:
+        assert s.sync_host() == 0
+    w64, w32 = (s.spec("omega_hat") for s in runs)
+    ef = rel_l2(w32, w64)
+    print(f"2000 steps: omega_hat rel L2 {ef:.2e}; worst E {worst_e:.1e}, Z {worst_z:.1e}",
+          flush=True)
+    assert np.isfinite(ef)
+    assert worst_e <= 1e-5 and worst_z <= 1e-5
+    for s in runs:
+        s.close()

@@ -33,6 +33,9 @@ def test_library_exports_every_declared_symbol(ns):
     missing = [n for n in sorted(decl) if not hasattr(L, n)]
     assert not missing, missing
     assert set(ns.EXPORTED) == decl
+    # a 64-bit context pointer passed without argtypes is truncated to a C int
+    untyped = [n for n in sorted(decl) if getattr(L, n).argtypes is None]
+    assert not untyped, untyped
 
 
 def test_nonpow2_or_missing_device_fails_loudly(ns):

@@ -341,3 +341,35 @@ def test_forcing_projection_and_injection_rate(oc):
     sel = (np.abs(raw) > 0) & (np.arange(17)[None, :] > 0)
     s = proj[sel] / raw[sel]
     assert np.allclose(s, s[0], rtol=1e-13)
+
+
+def test_config2_fixture_reproduces_and_decays(oc):
+    """The 1024^2 decaying-turbulence fixture (tests/golden/make_config2.py):
+    unit initial energy, monotone decay of energy and enstrophy without
+    forcing, shell counts of the spectrum, and the first 100 oracle steps
+    recomputed here."""
+    import os
+    from conftest import GOLDEN
+    G = np.load(os.path.join(GOLDEN, "config2_golden.npz"))
+    N, Re, seed, steps, every, klow = G["meta"]
+    N, seed, every = int(N), int(seed), int(every)
+    h = G["history"]
+    assert h[0, 3] == pytest.approx(1.0, rel=1e-12)
+    assert np.all(np.diff(h[:, 3]) < 0) and np.all(np.diff(h[:, 4]) < 0)
+    kx = np.arange(N // 2 + 1)
+    ky = np.where(np.arange(N) <= N // 2, np.arange(N), np.arange(N) - N)
+    bins = (np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2) + 0.5).astype(int)
+    counts = np.bincount(bins.ravel(), minlength=len(G["counts"]))[: len(G["counts"])]
+    assert np.array_equal(counts, G["counts"])
+    o = oc.Oracle(N, N, 2 * np.pi, 2 * np.pi, float(Re))
+    o.init_grids()
+    kmax = np.sqrt(2.0) * N / 2.0
+    o.hv_set(enabled=1, coefficient=1e-16 * kmax ** 8, order=8, cutoff_ratio=0.0)
+    o.d.cfl_factor = 0.5
+    o.init_random_phase(seed, 10.0, 1.0)
+    for _ in range(every):
+        o.d.dt = o.cfl()
+        o.advance_if()
+    assert o.d.t == pytest.approx(h[1, 1], rel=1e-13)
+    assert o.energy() == pytest.approx(h[1, 3], rel=1e-12)
+    assert o.enstrophy() == pytest.approx(h[1, 4], rel=1e-12)
