@@ -85,6 +85,39 @@ int nsd_profile_read(NsdContext *c, double *row_ms, double *col_ms, double *step
  * measure attainable bandwidth and to calibrate HBM byte counters. */
 int nsd_bandwidth_probe(NsdContext *c, size_t bytes, int reps, double *ms);
 
+/* ---- batches (ensembles).  No reference counterpart: the reference solves
+ * one field per process.  A batch context holds `members` independent fields
+ * of one grid size; every time-step kernel runs all of them in one launch
+ * (members along the grid's y dimension), each with its own t, dt, CFL
+ * reduction, 1/Re and forcing stream.  nsd_advance / nsd_advance_graph
+ * advance every member; nsd_state_upload/download address member 0;
+ * the compat operations below refuse batch contexts. */
+NsdContext *nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision, int members,
+                             char *err, size_t errlen);
+int nsd_batch_members(const NsdContext *c);
+/* per-member Re (NULL: keep the NsdParams value) and forcing seeds (NULL:
+ * keep NsdParams.force_seed; 0 maps to 0x123456789abcdef as forcing.c:99) */
+int nsd_set_member_params(NsdContext *c, const double *Re, const unsigned long long *force_seed);
+/* random-phase initial field of every member from its own seed */
+int nsd_batch_random_phase(NsdContext *c, const unsigned long long *seeds, double k_peak,
+                           double energy);
+/* E[m], Z[m] (Parseval, FP64 sums); t[m], dt[m], status[m]; either may be NULL */
+int nsd_batch_energy(NsdContext *c, double *E, double *Z);
+int nsd_batch_scalars(NsdContext *c, double *t, double *dt, int *status);
+int nsd_member_upload(NsdContext *c, int member, const double *omega_hat);
+int nsd_member_download(NsdContext *c, int member, double *omega_hat);
+
+/* One captured hipGraph that advances several batch contexts (different grid
+ * sizes, same precision) by one step per launch; each context's kernels run
+ * on its own stream inside the graph, so the sizes overlap on the device.
+ * The graph is re-captured when a context's parameters change. */
+typedef struct NsdMulti NsdMulti;
+NsdMulti *nsd_multi_create(NsdContext *const *contexts, int n);
+void nsd_multi_destroy(NsdMulti *m);
+const char *nsd_multi_error(const NsdMulti *m);
+int nsd_multi_advance(NsdMulti *m, int nsteps, int adaptive);
+int nsd_multi_synchronize(NsdMulti *m);
+
 /* ---- compat operations on host arrays (reference layouts, FP64) */
 int nsd_grids(NsdContext *c, double *x, double *y, double *kx, double *ky, double *k2, int *mask);
 int nsd_hv_weights(NsdContext *c, int enabled, double coefficient, int order, double cutoff_ratio,

@@ -63,6 +63,15 @@ EXPORTED = [
     "nsd_velocity", "nsd_derivatives", "nsd_nonlinear", "nsd_hv_apply", "nsd_hv_post_step",
     "nsd_taylor_green", "nsd_phys_reduce", "nsd_spec_reduce", "nsd_spectrum",
     "nsd_forcing_generate", "nsd_bandwidth_probe",
+    # ns_device.h (batches)
+    "nsd_create_batch", "nsd_batch_members", "nsd_set_member_params", "nsd_batch_random_phase",
+    "nsd_batch_energy", "nsd_batch_scalars", "nsd_member_upload", "nsd_member_download",
+    "nsd_multi_create", "nsd_multi_destroy", "nsd_multi_error", "nsd_multi_advance",
+    "nsd_multi_synchronize",
+    # ns_ensemble.h
+    "ns_ensemble_config_default", "ns_ensemble_create", "ns_ensemble_create_error",
+    "ns_ensemble_free", "ns_ensemble_members", "ns_ensemble_advance", "ns_ensemble_synchronize",
+    "ns_ensemble_diagnostics", "ns_ensemble_member_state", "ns_ensemble_last_error",
 ]
 
 
@@ -128,6 +137,13 @@ class NsdParams(C.Structure):  # ns_device.h
                 ("force_kmax", C.c_double), ("force_energy_target", C.c_double),
                 ("force_tau", C.c_double), ("force_rate", C.c_double),
                 ("force_seed", C.c_ulonglong)]
+
+
+class NSEnsembleConfig(C.Structure):  # ns_ensemble.h
+    _fields_ = [("Lx", C.c_double), ("Ly", C.c_double), ("precision", C.c_int),
+                ("scheme", C.c_int), ("cfl_factor", C.c_double), ("drag", C.c_double),
+                ("nu4", C.c_double), ("forcing_enabled", C.c_int), ("forcing", NSForcingConfig),
+                ("injection_rate", C.c_double), ("k_peak", C.c_double), ("energy", C.c_double)]
 
 
 _lib = None
@@ -269,6 +285,33 @@ def load():
                                   C.c_double, C.c_double, D, D, C.c_char_p, C.c_size_t],
                                  C.c_int),
     }
+    ULLP = C.POINTER(ull)
+    dev.update({
+        "nsd_create_batch": ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                              C.c_char_p, C.c_size_t], V),
+        "nsd_batch_members": ([V], C.c_int),
+        "nsd_set_member_params": ([V, D, ULLP], C.c_int),
+        "nsd_batch_random_phase": ([V, ULLP, C.c_double, C.c_double], C.c_int),
+        "nsd_batch_energy": ([V, D, D], C.c_int),
+        "nsd_batch_scalars": ([V, D, D, I], C.c_int),
+        "nsd_member_upload": ([V, C.c_int, D], C.c_int),
+        "nsd_member_download": ([V, C.c_int, D], C.c_int),
+        "nsd_multi_create": ([C.POINTER(V), C.c_int], V),
+        "nsd_multi_destroy": ([V], None),
+        "nsd_multi_error": ([V], C.c_char_p),
+        "nsd_multi_advance": ([V, C.c_int, C.c_int], C.c_int),
+        "nsd_multi_synchronize": ([V], C.c_int),
+        "ns_ensemble_config_default": ([C.POINTER(NSEnsembleConfig)], None),
+        "ns_ensemble_create": ([C.c_int, I, D, ULLP, C.POINTER(NSEnsembleConfig)], V),
+        "ns_ensemble_create_error": ([], C.c_char_p),
+        "ns_ensemble_free": ([V], None),
+        "ns_ensemble_members": ([V], C.c_int),
+        "ns_ensemble_advance": ([V, C.c_int], C.c_int),
+        "ns_ensemble_synchronize": ([V], C.c_int),
+        "ns_ensemble_diagnostics": ([V, D, D, D, I], C.c_int),
+        "ns_ensemble_member_state": ([V, C.c_int, D], C.c_int),
+        "ns_ensemble_last_error": ([V], C.c_char_p),
+    })
     for name, (args, res) in dev.items():
         getattr(L, name).argtypes = args
         getattr(L, name).restype = res
@@ -439,3 +482,69 @@ class Spectral:
     def ctx(self):
         """NsdContext* of this instance (first field of the device struct)."""
         return C.cast(self.d.device, C.POINTER(C.c_void_p))[0]
+
+
+class Ensemble:
+    """Independent solves of different sizes, Reynolds numbers and seeds
+    (ns_ensemble.h).  Keyword arguments set NSEnsembleConfig fields; `forcing`
+    takes (kmin, kmax, injection_rate)."""
+
+    def __init__(self, sizes, Re, seeds, forcing=None, **cfg):
+        self.L = load()
+        c = NSEnsembleConfig()
+        self.L.ns_ensemble_config_default(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        if forcing is not None:
+            c.forcing_enabled = 1
+            c.forcing.kmin, c.forcing.kmax = float(forcing[0]), float(forcing[1])
+            c.forcing.energy_target = 1.0
+            c.forcing.type = NS_FORCING_WHITE_NOISE
+            c.injection_rate = float(forcing[2])
+        self.cfg = c
+        self.sizes = [int(n) for n in sizes]
+        m = len(self.sizes)
+        n_arr = (C.c_int * m)(*self.sizes)
+        re_arr = (C.c_double * m)(*[float(r) for r in Re])
+        sd_arr = (C.c_ulonglong * m)(*[int(s) for s in seeds])
+        self.p = self.L.ns_ensemble_create(m, n_arr, re_arr, sd_arr, C.byref(c))
+        if not self.p:
+            raise RuntimeError("ns_ensemble_create: " + self.L.ns_ensemble_create_error().decode())
+        self.members = m
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.L.ns_ensemble_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"ensemble call failed ({rc}): "
+                               + self.L.ns_ensemble_last_error(self.p).decode())
+        return rc
+
+    def advance(self, nsteps):
+        return self.check(self.L.ns_ensemble_advance(self.p, nsteps))
+
+    def synchronize(self):
+        return self.check(self.L.ns_ensemble_synchronize(self.p))
+
+    def diagnostics(self):
+        m = self.members
+        t, e, z = np.zeros(m), np.zeros(m), np.zeros(m)
+        st = np.zeros(m, dtype=np.int32)
+        self.check(self.L.ns_ensemble_diagnostics(self.p, _dp(t), _dp(e), _dp(z),
+                                                  st.ctypes.data_as(c_int_p)))
+        return t, e, z, st
+
+    def state(self, member):
+        n = self.sizes[member]
+        out = np.zeros((n, n // 2 + 1), dtype=np.complex128)
+        self.check(self.L.ns_ensemble_member_state(self.p, member, _dp(out)))
+        return out

@@ -233,11 +233,34 @@ __global__ void k_spectrum(Geo G, const double* u, const double* v, double* ek, 
     }
 }
 
+// zero the reduction slots of every member's scalars (a kernel, not a
+// memset, so that it may sit inside a captured graph)
+__global__ void k_clear_red(DevScalars* sc, int nb) {
+    for (int m = threadIdx.x; m < nb; m += blockDim.x) {
+        for (int i = 0; i < 8; ++i) sc[m].red[i] = 0.0;
+        for (int i = 0; i < 4; ++i) sc[m].redmax[i] = 0ull;
+    }
+}
+
+// time, dt and forcing counter of every member
+__global__ void k_set_scalars(DevScalars* sc, int nb, int which, double t, double dt,
+                              unsigned long long fcount) {
+    for (int m = threadIdx.x; m < nb; m += blockDim.x) {
+        if (which & 1) {
+            sc[m].t = t;
+            sc[m].dt = dt;
+        }
+        if (which & 2) sc[m].fcount = fcount;
+    }
+}
+
 // Parseval sums over the compact state: E = 1/2 sum_full |w|^2/k^2,
 // Z = 1/2 sum_full |w|^2 (half-plane weights 1 at kx = 0, else 2).
 template <typename T>
 __global__ void k_state_energy(Geo G, const cx<T>* st, DevScalars* sc) {
     const size_t n = (size_t)G.nkx_keep * G.nky_keep;
+    st += blockIdx.y * n;  // member blockIdx.y of a batch
+    sc += blockIdx.y;
     double e = 0.0, z = 0.0;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
@@ -269,8 +292,11 @@ __device__ __forceinline__ double uniform_at(unsigned long long seed, unsigned l
 
 // Random-phase initial vorticity (build-defined, same definition as the oracle).
 template <typename T>
-__global__ void k_random_phase(Geo G, unsigned long long seed, double k_peak, cx<T>* st) {
+__global__ void k_random_phase(Geo G, unsigned long long seed, const unsigned long long* seeds,
+                               double k_peak, cx<T>* st) {
     const size_t n = (size_t)G.nkx_keep * G.nky_keep;
+    st += blockIdx.y * n;  // member blockIdx.y of a batch, with its own seed
+    if (seeds) seed = seeds[blockIdx.y];
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
         const int kxi = (int)(q / G.nky_keep), cy = (int)(q % G.nky_keep);
@@ -297,7 +323,8 @@ __global__ void k_random_phase(Geo G, unsigned long long seed, double k_peak, cx
 
 template <typename T>
 __global__ void k_scale_state(size_t n, cx<T>* st, const DevScalars* sc, double target) {
-    const double e = sc->red[0];
+    st += blockIdx.y * n;
+    const double e = sc[blockIdx.y].red[0];
     const T s = T(e > 0.0 ? sqrt(target / e) : 0.0);
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x)
@@ -323,10 +350,20 @@ struct ForceArgs {
     int box_T;           // 1: box in FP32, 0: FP64
     void* fbox;
     DevScalars* sc;
+    const unsigned long long* m_seed;  // per-member seeds of a batch (else seed)
 };
 
 __global__ void k_force(ForceArgs a) {
     const Geo& G = a.geo;
+    {  // member blockIdx.y of a batch: its own generator, OU state, box and scalars
+        const int mb = blockIdx.y;
+        const size_t boxn = (size_t)a.fbx * (2 * a.fby + 1);
+        a.sc += mb;
+        a.ou += (size_t)mb * 2 * a.A;
+        a.work += (size_t)mb * 4 * a.A;
+        a.fbox = (char*)a.fbox + (size_t)mb * boxn * (a.box_T ? 2 * sizeof(float) : 2 * sizeof(double));
+        if (a.m_seed) a.seed = a.m_seed[mb];
+    }
     const unsigned long long c = a.sc->fcount;
     const double dt = a.sc->dt;
     double alpha = 0.0, beta = 1.0;
@@ -516,6 +553,10 @@ unsigned grid_for(size_t n, unsigned bs = 256) {
 struct NsdContext {
     Geo G;
     int prec;
+    int nb = 1;                             // batch members (one field each)
+    double* m_inv_re = nullptr;             // per-member 1/Re (batches), device
+    unsigned long long* m_seed = nullptr;   // per-member forcing seeds (batches), device
+    unsigned long long* m_ic = nullptr;     // per-member initial-field seeds, device
     NsdParams P;
     Phys phys;
     hipStream_t stream = nullptr;
@@ -607,6 +648,10 @@ ColArgs<T> col_args(NsdContext* c) {
     a.sfd = (cx<T>*)c->sfd;
     a.fbox = (const cx<T>*)c->fbox;
     a.sc = c->sc;
+    a.bm = M_of(c);
+    a.bc = C_of(c);
+    a.bf = (size_t)c->fbx * (2 * c->fby + 1);
+    a.m_inv_re = c->m_inv_re;
     return a;
 }
 
@@ -619,6 +664,7 @@ RowArgs<T> row_args(NsdContext* c) {
     for (int i = 0; i < 4; ++i) a.mixed[i] = (const cx<T>*)c->mixed[i];
     a.nl = (cx<T>*)c->nl;
     a.sc = c->sc;
+    a.bm = M_of(c);
     return a;
 }
 
@@ -636,7 +682,7 @@ template <typename T>
 void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
     const int L = c->G.ny;
     if (a.mode == COL_STAGE || a.mode == COL_PRIME) {
-        launch_col_fused<T>(a, L, ncols, c->stream);
+        launch_col_fused<T>(a, L, ncols, c->nb, c->stream);
         return;
     }
     const int NT = threads_for(L);
@@ -660,7 +706,7 @@ template <typename T>
 void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items) {
     const int L = c->G.nx;
     if (a.mode == ROW_STAGE) {
-        launch_row_stage<T>(a, L, items, c->stream);
+        launch_row_stage<T>(a, L, items, c->nb, c->stream);
         return;
     }
     const int NT = threads_for(L);
@@ -749,7 +795,8 @@ void enqueue_force(NsdContext* c) {
     f.box_T = (sizeof(T) == 4) ? 1 : 0;
     f.fbox = c->fbox;
     f.sc = c->sc;
-    hipLaunchKernelGGL(k_force, dim3(1), dim3(256), 0, c->stream, f);
+    f.m_seed = c->m_seed;
+    hipLaunchKernelGGL(k_force, dim3(1, c->nb), dim3(256), 0, c->stream, f);
 }
 
 // one RK4 step: stage 0 computes dt (adaptive) before the forcing draw,
@@ -830,11 +877,11 @@ int op_advance_graph(NsdContext* c, int nsteps, int adaptive) {
 }
 
 template <typename T>
-int op_upload(NsdContext* c, const double* omega_hat) {
+int op_upload(NsdContext* c, const double* omega_hat, int member = 0) {
     if (ensure_staging(c)) return ST_MEM;
     if (up(c, c->stH[0], omega_hat, 2 * sizeof(double) * H_of(c))) return ST_MEM;
     hipLaunchKernelGGL(k_gather_state<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
-                       c->stH[0], (cx<T>*)c->state);
+                       c->stH[0], (cx<T>*)c->state + (size_t)member * C_of(c));
     if (op_prime<T>(c)) return ST_MEM;
     return sync(c);
 }
@@ -858,14 +905,16 @@ int op_sfd_upload(NsdContext* c, const double* sfd) {
 }
 
 template <typename T>
-int op_random_phase(NsdContext* c, unsigned long long seed, double k_peak, double energy) {
-    hipLaunchKernelGGL(k_random_phase<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
-                       seed, k_peak, (cx<T>*)c->state);
-    CK(hipMemsetAsync(c->sc->red, 0, sizeof(c->sc->red), c->stream));
-    hipLaunchKernelGGL(k_state_energy<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+int op_random_phase(NsdContext* c, unsigned long long seed, const unsigned long long* seeds,
+                    double k_peak, double energy) {
+    const dim3 grid(grid_for(C_of(c)), c->nb);
+    hipLaunchKernelGGL(k_random_phase<T>, grid, dim3(256), 0, c->stream, c->G, seed, seeds,
+                       k_peak, (cx<T>*)c->state);
+    hipLaunchKernelGGL(k_clear_red, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb);
+    hipLaunchKernelGGL(k_state_energy<T>, grid, dim3(256), 0, c->stream, c->G,
                        (const cx<T>*)c->state, c->sc);
-    hipLaunchKernelGGL(k_scale_state<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream,
-                       C_of(c), (cx<T>*)c->state, (const DevScalars*)c->sc, energy);
+    hipLaunchKernelGGL(k_scale_state<T>, grid, dim3(256), 0, c->stream, C_of(c),
+                       (cx<T>*)c->state, (const DevScalars*)c->sc, energy);
     if (op_prime<T>(c)) return ST_MEM;
     return sync(c);
 }
@@ -950,15 +999,18 @@ int op_state_derived(NsdContext* c, double* psi_hat, double* u_hat, double* v_ha
     return sync(c);
 }
 
+// E and Z of every member: out[2 m], out[2 m + 1]
 template <typename T>
-int op_state_energy(NsdContext* c, double out[2]) {
-    CK(hipMemsetAsync(c->sc->red, 0, sizeof(c->sc->red), c->stream));
-    hipLaunchKernelGGL(k_state_energy<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
-                       (const cx<T>*)c->state, c->sc);
-    double red[8];
-    if (down(c, red, c->sc->red, sizeof(red)) || sync(c)) return ST_MEM;
-    out[0] = red[0];
-    out[1] = red[1];
+int op_state_energy(NsdContext* c, double* out) {
+    hipLaunchKernelGGL(k_clear_red, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb);
+    hipLaunchKernelGGL(k_state_energy<T>, dim3(grid_for(C_of(c)), c->nb), dim3(256), 0, c->stream,
+                       c->G, (const cx<T>*)c->state, c->sc);
+    std::vector<DevScalars> h(c->nb);
+    if (lastk(c) || down(c, h.data(), c->sc, sizeof(DevScalars) * c->nb) || sync(c)) return ST_MEM;
+    for (int m = 0; m < c->nb; ++m) {
+        out[2 * m] = h[m].red[0];
+        out[2 * m + 1] = h[m].red[1];
+    }
     return ST_OK;
 }
 
@@ -969,18 +1021,138 @@ int restore_mixed(NsdContext* c) {
     return op_prime<T>(c);
 }
 
+// ----------------------------------------------------------------- batches
+
+int batch_refused(NsdContext* c) {
+    c->err = "compat operations need a single-member context";
+    return ST_PARAM;
+}
+
+}  // namespace
+
+// One graph for a whole ensemble: the step of every batch context, each on
+// its own stream, forked from and joined back into the first context's
+// stream during capture, so that one hipGraphLaunch advances all of them.
+struct NsdMulti {
+    std::vector<NsdContext*> cs;
+    std::vector<unsigned long long> gens;
+    std::vector<hipEvent_t> ev;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int adaptive = -1;
+    std::string err;
+};
+
+namespace {
+
+int multi_capture(NsdMulti* mh, int adaptive) {
+    NsdContext* c0 = mh->cs[0];
+    if (mh->gexec) (void)hipGraphExecDestroy(mh->gexec);
+    if (mh->graph) (void)hipGraphDestroy(mh->graph);
+    mh->gexec = nullptr;
+    mh->graph = nullptr;
+    const size_t n = mh->cs.size();
+    hipError_t e = hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = hipEventRecord(mh->ev[0], c0->stream);
+    for (size_t i = 1; i < n && e == hipSuccess; ++i)
+        e = hipStreamWaitEvent(mh->cs[i]->stream, mh->ev[0], 0);
+    for (size_t i = 0; i < n && e == hipSuccess; ++i) {
+        NsdContext* c = mh->cs[i];
+        const int prof = c->prof;
+        c->prof = 0;
+        if (c->prec == NSD_FP32) enqueue_step<float>(c, adaptive);
+        else enqueue_step<double>(c, adaptive);
+        c->prof = prof;
+    }
+    for (size_t i = 1; i < n && e == hipSuccess; ++i) {
+        e = hipEventRecord(mh->ev[i], mh->cs[i]->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c0->stream, mh->ev[i], 0);
+    }
+    hipError_t e2 = hipStreamEndCapture(c0->stream, &mh->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&mh->gexec, mh->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        mh->err = std::string("ensemble graph capture: ") + hipGetErrorString(e);
+        return ST_MEM;
+    }
+    mh->adaptive = adaptive;
+    for (size_t i = 0; i < n; ++i) mh->gens[i] = mh->cs[i]->gen;
+    return ST_OK;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- extern C
 
 extern "C" {
 
+NsdMulti* nsd_multi_create(NsdContext* const* cs, int n) {
+    if (!cs || n < 1) return nullptr;
+    NsdMulti* mh = new NsdMulti();
+    for (int i = 0; i < n; ++i) {
+        if (!cs[i] || cs[i]->prec != cs[0]->prec) {
+            delete mh;
+            return nullptr;
+        }
+        mh->cs.push_back(cs[i]);
+    }
+    mh->gens.assign(n, ~0ull);
+    mh->ev.assign(n, nullptr);
+    for (int i = 0; i < n; ++i)
+        if (hipEventCreateWithFlags(&mh->ev[i], hipEventDisableTiming) != hipSuccess) {
+            nsd_multi_destroy(mh);
+            return nullptr;
+        }
+    return mh;
+}
+
+void nsd_multi_destroy(NsdMulti* mh) {
+    if (!mh) return;
+    for (NsdContext* c : mh->cs) (void)hipStreamSynchronize(c->stream);
+    if (mh->gexec) (void)hipGraphExecDestroy(mh->gexec);
+    if (mh->graph) (void)hipGraphDestroy(mh->graph);
+    for (hipEvent_t e : mh->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete mh;
+}
+
+const char* nsd_multi_error(const NsdMulti* mh) { return mh ? mh->err.c_str() : "null handle"; }
+
+int nsd_multi_advance(NsdMulti* mh, int nsteps, int adaptive) {
+    if (!mh || nsteps < 0) return ST_PARAM;
+    adaptive = adaptive ? 1 : 0;
+    bool stale = !mh->gexec || mh->adaptive != adaptive;
+    for (size_t i = 0; i < mh->cs.size(); ++i) stale = stale || mh->gens[i] != mh->cs[i]->gen;
+    if (stale && multi_capture(mh, adaptive)) return ST_MEM;
+    for (int s = 0; s < nsteps; ++s)
+        if (hipGraphLaunch(mh->gexec, mh->cs[0]->stream) != hipSuccess) {
+            mh->err = "hipGraphLaunch failed";
+            return ST_MEM;
+        }
+    return ST_OK;
+}
+
+int nsd_multi_synchronize(NsdMulti* mh) {
+    if (!mh) return ST_PARAM;
+    if (hipStreamSynchronize(mh->cs[0]->stream) != hipSuccess) {
+        mh->err = "hipStreamSynchronize failed";
+        return ST_MEM;
+    }
+    return ST_OK;
+}
+
 NsdContext* nsd_create(int nx, int ny, double Lx, double Ly, int precision, char* err,
                        size_t errlen) {
+    return nsd_create_batch(nx, ny, Lx, Ly, precision, 1, err, errlen);
+}
+
+NsdContext* nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision, int members,
+                             char* err, size_t errlen) {
     auto fail = [&](const char* m) -> NsdContext* {
         if (err && errlen) snprintf(err, errlen, "%s", m);
         return nullptr;
     };
+    if (members < 1 || members > 65535) return fail("members must be in [1, 65535]");
     auto pow2 = [](int n) { return n >= 16 && n <= 8192 && (n & (n - 1)) == 0; };
     if (!pow2(nx) || !pow2(ny)) return fail("nx and ny must be powers of two in [16, 8192]");
     if (!(Lx > 0.0) || !(Ly > 0.0)) return fail("Lx and Ly must be positive");
@@ -988,6 +1160,7 @@ NsdContext* nsd_create(int nx, int ny, double Lx, double Ly, int precision, char
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
     NsdContext* c = new NsdContext();
+    c->nb = members;
     Geo& G = c->G;
     G.nx = nx;
     G.ny = ny;
@@ -1014,15 +1187,16 @@ NsdContext* nsd_create(int nx, int ny, double Lx, double Ly, int precision, char
         return false;
     };
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return nullptr;
-    const size_t C = C_of(c) * c->esz, M = M_of(c) * c->esz;
+    const size_t C = C_of(c) * c->esz * members, M = M_of(c) * c->esz * members;
+    const size_t S = sizeof(DevScalars) * members;
     if (bad(hipMalloc(&c->state, C)) || bad(hipMalloc(&c->acc, C)) ||
         bad(hipMalloc(&c->stagebuf, C)) || bad(hipMalloc(&c->sfd, C)) ||
-        bad(hipMalloc(&c->nl, M)) || bad(hipMalloc(&c->sc, sizeof(DevScalars))))
+        bad(hipMalloc(&c->nl, M)) || bad(hipMalloc(&c->sc, S)))
         return nullptr;
     for (int i = 0; i < 4; ++i)
         if (bad(hipMalloc(&c->mixed[i], M))) return nullptr;
     if (bad(hipMemset(c->state, 0, C)) || bad(hipMemset(c->sfd, 0, C)) ||
-        bad(hipMemset(c->sc, 0, sizeof(DevScalars))))
+        bad(hipMemset(c->sc, 0, S)))
         return nullptr;
     // twiddle tables
     for (int d = 0; d < 2; ++d) {
@@ -1068,7 +1242,7 @@ void nsd_destroy(NsdContext* c) {
     void* bufs[] = {c->state, c->acc, c->stagebuf, c->sfd, c->nl, c->twx, c->twy, c->sc,
                     c->mixed[0], c->mixed[1], c->mixed[2], c->mixed[3], c->stH[0], c->stH[1],
                     c->stH[2], c->stH[3], c->stR[0], c->stR[1], c->stR[2], c->stR[3], c->fmodes,
-                    c->fpartner, c->fou, c->fwork, c->fbox};
+                    c->fpartner, c->fou, c->fwork, c->fbox, c->m_inv_re, c->m_seed, c->m_ic};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1150,13 +1324,14 @@ int nsd_set_params(NsdContext* c, const NsdParams* p) {
         const size_t box = (size_t)fbx * (2 * fby + 1);
         CK(hipMalloc(&c->fmodes, sizeof(int) * A));
         CK(hipMalloc(&c->fpartner, sizeof(int) * A));
-        CK(hipMalloc(&c->fou, 2 * sizeof(double) * A));
-        CK(hipMalloc(&c->fwork, 4 * sizeof(double) * A));
-        CK(hipMalloc(&c->fbox, c->esz * box));
+        const size_t nb = (size_t)c->nb;
+        CK(hipMalloc(&c->fou, 2 * sizeof(double) * A * nb));
+        CK(hipMalloc(&c->fwork, 4 * sizeof(double) * A * nb));
+        CK(hipMalloc(&c->fbox, c->esz * box * nb));
         CK(hipMemcpy(c->fmodes, modes.data(), sizeof(int) * A, hipMemcpyHostToDevice));
         CK(hipMemcpy(c->fpartner, partner.data(), sizeof(int) * A, hipMemcpyHostToDevice));
-        CK(hipMemset(c->fou, 0, 2 * sizeof(double) * A));
-        CK(hipMemset(c->fbox, 0, c->esz * box));
+        CK(hipMemset(c->fou, 0, 2 * sizeof(double) * A * nb));
+        CK(hipMemset(c->fbox, 0, c->esz * box * nb));
         f.force_on = 1;
         f.fbx = fbx;
         f.fby = fby;
@@ -1188,13 +1363,14 @@ int nsd_sfd_download(NsdContext* c, double* s) {
 
 int nsd_state_random_phase(NsdContext* c, unsigned long long seed, double k_peak, double energy) {
     if (!c || !(k_peak > 0.0) || !(energy >= 0.0)) return ST_PARAM;
-    return TYPED(op_random_phase, c, seed, k_peak, energy);
+    return TYPED(op_random_phase, c, seed, nullptr, k_peak, energy);
 }
 
 int nsd_set_time(NsdContext* c, double t, double dt) {
     if (!c) return ST_PARAM;
-    double v[2] = {t, dt};
-    CK(hipMemcpyAsync(c->sc, v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb, 1, t, dt,
+                       0ull);
+    if (lastk(c)) return ST_MEM;
     return sync(c);
 }
 
@@ -1211,7 +1387,9 @@ int nsd_get_time(NsdContext* c, double* t, double* dt, int* status) {
 
 int nsd_set_forcing_counter(NsdContext* c, unsigned long long calls) {
     if (!c) return ST_PARAM;
-    CK(hipMemcpyAsync(&c->sc->fcount, &calls, sizeof(calls), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb, 2, 0.0, 0.0,
+                       calls);
+    if (lastk(c)) return ST_MEM;
     return sync(c);
 }
 
@@ -1241,7 +1419,83 @@ int nsd_state_derived(NsdContext* c, double* psi_hat, double* u_hat, double* v_h
 
 int nsd_state_energy(NsdContext* c, double out[2]) {
     if (!c || !out) return ST_PARAM;
+    if (c->nb != 1) {
+        c->err = "nsd_state_energy: batch context (use nsd_batch_energy)";
+        return ST_PARAM;
+    }
     return TYPED(op_state_energy, c, out);
+}
+
+int nsd_batch_members(const NsdContext* c) { return c ? c->nb : 0; }
+
+int nsd_set_member_params(NsdContext* c, const double* Re, const unsigned long long* force_seed) {
+    if (!c) return ST_PARAM;
+    const size_t nb = (size_t)c->nb;
+    if (Re) {
+        std::vector<double> inv(nb);
+        for (size_t m = 0; m < nb; ++m) {
+            if (!(Re[m] > 0.0)) {
+                c->err = "member Re must be positive";
+                return ST_PARAM;
+            }
+            inv[m] = 1.0 / Re[m];
+        }
+        if (!c->m_inv_re) CK(hipMalloc(&c->m_inv_re, sizeof(double) * nb));
+        CK(hipMemcpy(c->m_inv_re, inv.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+    }
+    if (force_seed) {
+        std::vector<unsigned long long> sd(force_seed, force_seed + nb);
+        for (auto& v : sd)
+            if (v == 0) v = 0x123456789abcdefull;  // forcing.c:99
+        if (!c->m_seed) CK(hipMalloc(&c->m_seed, sizeof(unsigned long long) * nb));
+        CK(hipMemcpy(c->m_seed, sd.data(), sizeof(unsigned long long) * nb, hipMemcpyHostToDevice));
+    }
+    c->gen++;  // kernel arguments changed: re-capture the step graph
+    return ST_OK;
+}
+
+int nsd_batch_random_phase(NsdContext* c, const unsigned long long* seeds, double k_peak,
+                           double energy) {
+    if (!c || !seeds || !(k_peak > 0.0) || !(energy >= 0.0)) return ST_PARAM;
+    const size_t nb = (size_t)c->nb;
+    if (!c->m_ic) CK(hipMalloc(&c->m_ic, sizeof(unsigned long long) * nb));
+    CK(hipMemcpy(c->m_ic, seeds, sizeof(unsigned long long) * nb, hipMemcpyHostToDevice));
+    return TYPED(op_random_phase, c, 0ull, c->m_ic, k_peak, energy);
+}
+
+int nsd_batch_energy(NsdContext* c, double* E, double* Z) {
+    if (!c) return ST_PARAM;
+    std::vector<double> out(2 * (size_t)c->nb);
+    int rc = TYPED(op_state_energy, c, out.data());
+    if (rc) return rc;
+    for (int m = 0; m < c->nb; ++m) {
+        if (E) E[m] = out[2 * m];
+        if (Z) Z[m] = out[2 * m + 1];
+    }
+    return ST_OK;
+}
+
+int nsd_batch_scalars(NsdContext* c, double* t, double* dt, int* status) {
+    if (!c) return ST_PARAM;
+    std::vector<DevScalars> h(c->nb);
+    if (down(c, h.data(), c->sc, sizeof(DevScalars) * c->nb) || sync(c)) return ST_MEM;
+    for (int m = 0; m < c->nb; ++m) {
+        if (t) t[m] = h[m].t;
+        if (dt) dt[m] = h[m].dt;
+        if (status) status[m] = h[m].status;
+    }
+    return ST_OK;
+}
+
+int nsd_member_upload(NsdContext* c, int member, const double* w) {
+    if (!c || !w || member < 0 || member >= c->nb) return ST_PARAM;
+    return TYPED(op_upload, c, w, member);
+}
+
+int nsd_member_download(NsdContext* c, int member, double* w) {
+    if (!c || !w || member < 0 || member >= c->nb) return ST_PARAM;
+    void* src = (char*)c->state + (size_t)member * C_of(c) * c->esz;
+    return TYPED(op_download, c, w, src);
 }
 
 int nsd_synchronize(NsdContext* c) {
@@ -1353,6 +1607,7 @@ int nsd_hv_weights(NsdContext* c, int enabled, double coefficient, int order, do
 
 int nsd_physical_to_spectral(NsdContext* c, const double* field, double* field_hat) {
     if (!c || !field || !field_hat) return ST_PARAM;
+    if (c->nb != 1) return batch_refused(c);
     if (ensure_staging(c) || up(c, c->stR[0], field, sizeof(double) * R_of(c))) return ST_MEM;
     if (TYPED(op_forward, c, c->stR[0], c->stH[0], 1.0)) return ST_MEM;
     if (down(c, field_hat, c->stH[0], 2 * sizeof(double) * H_of(c))) return ST_MEM;
@@ -1362,6 +1617,7 @@ int nsd_physical_to_spectral(NsdContext* c, const double* field, double* field_h
 
 int nsd_spectral_to_physical(NsdContext* c, const double* field_hat, double* field) {
     if (!c || !field_hat || !field) return ST_PARAM;
+    if (c->nb != 1) return batch_refused(c);
     if (ensure_staging(c) || up(c, c->stH[0], field_hat, 2 * sizeof(double) * H_of(c)))
         return ST_MEM;
     const int ops[1] = {OP_ID0};
@@ -1396,6 +1652,7 @@ int nsd_velocity(NsdContext* c, const double* w, double* psi, double* u, double*
 int nsd_derivatives(NsdContext* c, const double* w, double* psi, double* uh, double* vh, double* u,
                     double* v, double* om) {
     if (!c || !w) return ST_PARAM;
+    if (c->nb != 1) return batch_refused(c);
     const size_t hb = 2 * sizeof(double) * H_of(c), rb = sizeof(double) * R_of(c);
     if (ensure_staging(c) || up(c, c->stH[0], w, hb)) return ST_MEM;
     hipLaunchKernelGGL(k_velocity_ref, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
@@ -1412,6 +1669,7 @@ int nsd_derivatives(NsdContext* c, const double* w, double* psi, double* uh, dou
 
 int nsd_nonlinear(NsdContext* c, const double* w, const double* psi, double* out) {
     if (!c || !w || !psi || !out) return ST_PARAM;
+    if (c->nb != 1) return batch_refused(c);
     const size_t hb = 2 * sizeof(double) * H_of(c);
     if (ensure_staging(c) || up(c, c->stH[0], w, hb) || up(c, c->stH[1], psi, hb)) return ST_MEM;
     if (TYPED(op_nonlinear, c, c->stH[0], c->stH[1], c->stH[2])) return ST_MEM;

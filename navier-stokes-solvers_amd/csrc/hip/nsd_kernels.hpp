@@ -271,6 +271,10 @@ struct ColArgs {
     const cx<T>* in0;  // compact inputs for COL_INV (if !in_ref)
     double* out_ref;   // COL_FWD output, FP64 reference layout, all nkx columns
     DevScalars* sc;
+    // batches: member blockIdx.y works on fields offset by these element
+    // strides, on scalars sc[blockIdx.y] and, if set, its own 1/Re
+    size_t bm, bc, bf;
+    const double* m_inv_re;
 };
 
 template <typename T>
@@ -305,6 +309,21 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     const Geo& G = a.geo;
+    if constexpr (MODE == COL_STAGE || MODE == COL_PRIME) {
+        const size_t mb = blockIdx.y;
+        a.nl += mb * a.bm;
+        for (int i = 0; i < 4; ++i) a.mixed[i] += mb * a.bm;
+        a.state += mb * a.bc;
+        a.acc += mb * a.bc;
+        a.stagebuf += mb * a.bc;
+        a.sfd += mb * a.bc;
+        a.fbox += mb * a.bf;
+        a.sc += mb;
+        if (a.m_inv_re) {
+            a.phys.inv_Re = a.m_inv_re[mb];
+            a.phys.Re = 1.0 / a.phys.inv_Re;
+        }
+    }
     const Line g = make_line(LC > 0 ? LC : G.ny);
     const int TPL = LC > 0 ? LC / P : g.TPL;
     const int Ll = LC > 0 ? LC : g.L;
@@ -552,6 +571,7 @@ struct RowArgs {
     T* out[4];             // ROW_C2R outputs (T [ny][nx]) otherwise
     const double* in_ref;  // ROW_R2C input (FP64 [ny][nx])
     DevScalars* sc;
+    size_t bm;             // batches: mixed-field stride of member blockIdx.y
 };
 
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
@@ -699,6 +719,12 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     cx<T> v[P];
+    if constexpr (MODE == ROW_STAGE) {
+        const size_t mb = blockIdx.y;
+        for (int i = 0; i < 4; ++i) a.mixed[i] += mb * a.bm;
+        a.nl += mb * a.bm;
+        a.sc += mb;
+    }
 
     if constexpr (MODE == ROW_C2R) {
         const int y = item;

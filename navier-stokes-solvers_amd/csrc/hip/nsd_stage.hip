@@ -17,28 +17,31 @@ constexpr int nt_for() {
 }
 
 template <typename T, int LOG>
-void row_stage_L(const RowArgs<T>& a, int items, hipStream_t st) {
+void row_stage_L(const RowArgs<T>& a, int items, int members, hipStream_t st) {
     constexpr int L = 1 << LOG;
     constexpr int NT = nt_for<L>();
     constexpr int lines = NT * P / L;
     const int blocks = (items + lines - 1) / lines;
     const size_t lds = (size_t)lines * line_regions<T, L>() * (L + L / 16) * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L>), dim3(blocks), dim3(NT), lds, st, a);
+    hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L>), dim3(blocks, members), dim3(NT), lds, st, a);
 }
 
 template <typename T, int LOG>
-void col_fused_L(const ColArgs<T>& a, int ncols, hipStream_t st) {
+void col_fused_L(const ColArgs<T>& a, int ncols, int members, hipStream_t st) {
     constexpr int L = 1 << LOG;
     constexpr int NT = nt_for<L>();
     constexpr int lines = NT * P / L;
     const int blocks = (ncols + lines - 1) / lines;
     const size_t lds = (size_t)lines * line_regions<T, L>() * (L + L / 16) * sizeof(cx<T>);
     if (a.mode == COL_PRIME)
-        hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L>), dim3(blocks), dim3(NT), lds, st, a);
+        hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L>), dim3(blocks, members), dim3(NT), lds, st,
+                           a);
     else if (a.phys.scheme == 0)
-        hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 0, L>), dim3(blocks), dim3(NT), lds, st, a);
+        hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 0, L>), dim3(blocks, members), dim3(NT), lds,
+                           st, a);
     else
-        hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 1, L>), dim3(blocks), dim3(NT), lds, st, a);
+        hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 1, L>), dim3(blocks, members), dim3(NT), lds,
+                           st, a);
 }
 
 template <typename K>
@@ -68,10 +71,10 @@ int log2i(int L) {
 #define NSD_FOR_LOGS(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
 
 template <typename T>
-void launch_row_stage(const RowArgs<T>& a, int L, int items, hipStream_t st) {
+void launch_row_stage(const RowArgs<T>& a, int L, int items, int members, hipStream_t st) {
     switch (log2i(L)) {
 #define X(n) \
-    case n: row_stage_L<T, n>(a, items, st); break;
+    case n: row_stage_L<T, n>(a, items, members, st); break;
         NSD_FOR_LOGS(X)
 #undef X
     default: break;
@@ -79,10 +82,10 @@ void launch_row_stage(const RowArgs<T>& a, int L, int items, hipStream_t st) {
 }
 
 template <typename T>
-void launch_col_fused(const ColArgs<T>& a, int L, int ncols, hipStream_t st) {
+void launch_col_fused(const ColArgs<T>& a, int L, int ncols, int members, hipStream_t st) {
     switch (log2i(L)) {
 #define X(n) \
-    case n: col_fused_L<T, n>(a, ncols, st); break;
+    case n: col_fused_L<T, n>(a, ncols, members, st); break;
         NSD_FOR_LOGS(X)
 #undef X
     default: break;
@@ -96,8 +99,8 @@ void stage_lds_limits(int bytes) {
 #undef X
 }
 
-template void launch_row_stage<NSD_T>(const RowArgs<NSD_T>&, int, int, hipStream_t);
-template void launch_col_fused<NSD_T>(const ColArgs<NSD_T>&, int, int, hipStream_t);
+template void launch_row_stage<NSD_T>(const RowArgs<NSD_T>&, int, int, int, hipStream_t);
+template void launch_col_fused<NSD_T>(const ColArgs<NSD_T>&, int, int, int, hipStream_t);
 template void stage_lds_limits<NSD_T>(int);
 
 }  // namespace nsd

@@ -1,0 +1,118 @@
+"""GPU tests of the ensemble path (batched kernels, one graph for all sizes).
+
+Each member of an ensemble must follow exactly the run of a standalone solver
+with the same size, Reynolds number and seed: the batched kernels execute the
+same arithmetic on offset buffers, so the states agree to round-off of the
+two host-side coefficient formulas (1e-12 relative L2 in FP64, 1e-5 in FP32).
+The FP32 ensemble is also compared with the FP64 ensemble member by member
+(the per-member FP64 deviation), and the FP64 members of the forced
+integrating-factor physics with the CPU oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2 * np.pi
+NU4 = 1e-16
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu] {request.node.name}", flush=True)
+
+
+def single_run(ns, n, Re, seed, steps, fp32, forcing):
+    s = ns.Spectral(n, n, TWO_PI, TWO_PI, Re)
+    s.init(taylor_green=False)
+    kmax = np.sqrt(2.0) * n / 2.0
+    s.hv_set(enabled=1, coefficient=NU4 * kmax ** 8, order=8, cutoff_ratio=0.0)
+    s.set_precision(fp32)
+    s.set_scheme(True)
+    s.set_drag(0.1)
+    if forcing:
+        s.set_forcing(4.0, 6.0, 1e-3, seed=seed)
+    s.d.cfl_factor = 0.5
+    s.random_phase(seed, k_peak=10.0, energy=1.0)
+    s.sync_device()
+    assert s.advance_n(steps, True) == 0
+    assert s.sync_host() == 0
+    w, t = s.spec("omega_hat").copy(), s.d.t
+    s.close()
+    return w, t
+
+
+def make_ensemble(ns, sizes, Re, seeds, fp32, forcing=True):
+    return ns.Ensemble(sizes, Re, seeds, forcing=(4.0, 6.0, 1e-3) if forcing else None,
+                       precision=ns.NS_PRECISION_FP32 if fp32 else ns.NS_PRECISION_FP64,
+                       scheme=ns.NS_SCHEME_IF_RK4, cfl_factor=0.5, drag=0.1, nu4=NU4,
+                       k_peak=10.0, energy=1.0)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_members_match_standalone_runs(ns, fp32):
+    sizes = [32, 64, 32, 128, 64, 32]
+    Re = [100.0, 1e3, 5e3, 300.0, 2e4, 1e5]
+    seeds = [1, 2, 3, 4, 5, 0]
+    steps = 25
+    e = make_ensemble(ns, sizes, Re, seeds, fp32)
+    e.advance(steps)
+    t, E, Z, st = e.diagnostics()
+    assert np.all(st == 0)
+    tol = 1e-5 if fp32 else 1e-12
+    for m, (n, r, sd) in enumerate(zip(sizes, Re, seeds)):
+        w_ref, t_ref = single_run(ns, n, r, sd, steps, fp32, True)
+        w = e.state(m)
+        err = rel_l2(w, w_ref)
+        print(f"member {m}: n={n} Re={r:g} seed={sd} t={t[m]:.6f} rel L2 vs standalone {err:.1e}",
+              flush=True)
+        assert t[m] == pytest.approx(t_ref, rel=1e-12)
+        assert err <= tol
+    e.close()
+
+
+def test_fp32_members_vs_fp64(ns):
+    rng = np.random.default_rng(7)
+    sizes = list(rng.choice([64, 128, 256], size=12))
+    Re = list(np.exp(rng.uniform(np.log(1e3), np.log(1e5), size=12)))
+    seeds = list(range(100, 112))
+    ens = [make_ensemble(ns, sizes, Re, seeds, fp32) for fp32 in (False, True)]
+    for e in ens:
+        e.advance(1)
+    one = [rel_l2(ens[1].state(m), ens[0].state(m)) for m in range(len(sizes))]
+    for e in ens:
+        e.advance(199)
+    d64, d32 = ens[0].diagnostics(), ens[1].diagnostics()
+    for m in range(len(sizes)):
+        dev = rel_l2(ens[1].state(m), ens[0].state(m))
+        de = abs(d32[1][m] / d64[1][m] - 1)
+        print(f"member {m}: n={sizes[m]} Re={Re[m]:.3g} FP64 deviation after 1 step {one[m]:.1e}, "
+              f"after 200 steps {dev:.1e}, energy {de:.1e}", flush=True)
+        assert one[m] <= 1e-5
+        assert de <= 1e-4
+    for e in ens:
+        e.close()
+
+
+def test_fp64_member_vs_oracle(ns, oc):
+    """One member of a mixed-size FP64 ensemble against the oracle's forced
+    integrating-factor run (the forcing draws of the member's seed)."""
+    n, Re, seed, steps = 64, 1e4, 9, 30
+    e = make_ensemble(ns, [32, n, 128], [500.0, Re, 2e3], [1, seed, 3], False)
+    e.advance(steps)
+    w = e.state(1)
+    e.close()
+    o = oc.Oracle(n, n, TWO_PI, TWO_PI, Re)
+    o.init_grids()
+    o.hv_set(enabled=1, coefficient=NU4 * (np.sqrt(2.0) * n / 2) ** 8, order=8, cutoff_ratio=0.0)
+    o.d.drag = 0.1
+    o.d.cfl_factor = 0.5
+    o.init_random_phase(seed, k_peak=10.0)
+    f = oc.OracleForcing(o.k2, n // 2 + 1, n, 4.0, 6.0, energy_target=1.0, seed=seed)
+    for _ in range(steps):
+        o.d.dt = o.cfl()
+        o.advance_if(o.project_forcing(f.generate(o.d.dt), 1e-3))
+    err = rel_l2(w, o.spec("omega_hat"))
+    print(f"member vs oracle after {steps} steps: rel L2 {err:.1e}", flush=True)
+    assert err <= 1e-11
