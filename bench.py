@@ -106,6 +106,40 @@ def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0):
     return res
 
 
+def ensemble_workload(members=256, seed=2024):
+    """Sizes uniform over {128, 256, 512, 1024}, nu log-uniform in [1e-5, 1e-3],
+    one seed per member; forcing, drag and del^8 hyperviscosity as the 4096^2
+    workload, FP32 integrating-factor stages."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 256, 512, 1024], size=members)
+    nu = np.exp(rng.uniform(np.log(1e-5), np.log(1e-3), size=members))
+    seeds = np.arange(1, members + 1)
+    return sizes, 1.0 / nu, seeds
+
+
+def time_ensemble(ns, steps, warmup, members=256):
+    sizes, Re, seeds = ensemble_workload(members)
+    e = ns.Ensemble(sizes, Re, seeds, forcing=(4.0, 6.0, 1e-3), precision=ns.NS_PRECISION_FP32,
+                    scheme=ns.NS_SCHEME_IF_RK4, cfl_factor=0.5, drag=0.1, nu4=1e-16,
+                    k_peak=10.0, energy=1.0)
+    e.advance(warmup)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.advance(steps)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    t, E, Z, st = e.diagnostics()
+    e.close()
+    pts = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 2))
+    return {"workload": f"{members} members, sizes uniform over 128..1024, nu log-uniform "
+                        "[1e-5, 1e-3], FP32 IF-RK4, forcing/drag/del^8 as the 4096^2 workload",
+            "members_steps_per_s": round(members * steps / wall, 1),
+            "gpoint_steps_per_s": round(pts * steps / wall / 1e9, 3),
+            "ms_per_ensemble_step": round(1e3 * wall / steps, 4),
+            "members_per_size": {int(n): int(np.sum(sizes == n)) for n in (128, 256, 512, 1024)},
+            "all_status_ok": bool(np.all(st == 0))}
+
+
 def cpu_baseline(n):
     """Oracle (reference algorithm, project FFT, OpenMP) on this host: one step
     of the same forced integrating-factor workload at FP64."""
@@ -137,6 +171,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 256-member mixed-size ensemble")
     ap.add_argument("--no-compat", action="store_true",
                     help="skip timing the PCIe-bound reference entry point ns_spectral_advance_rk4")
     ap.add_argument("--probe", action="store_true",
@@ -204,6 +240,8 @@ def main():
         out["fp64"] = {"value": round(n * n / (r64["ms_per_step"] * 1e-3) / 1e9, 3),
                        "ms_per_step": round(r64["ms_per_step"], 4),
                        "row_ms": round(r64["row_ms"], 4), "col_ms": round(r64["col_ms"], 4)}
+    if not a.no_ensemble:
+        out["ensemble"] = time_ensemble(ns, max(a.steps // 2, 5), 3)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

@@ -309,20 +309,21 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     const Geo& G = a.geo;
-    if constexpr (MODE == COL_STAGE || MODE == COL_PRIME) {
-        const size_t mb = blockIdx.y;
-        a.nl += mb * a.bm;
-        for (int i = 0; i < 4; ++i) a.mixed[i] += mb * a.bm;
-        a.state += mb * a.bc;
-        a.acc += mb * a.bc;
-        a.stagebuf += mb * a.bc;
-        a.sfd += mb * a.bc;
-        a.fbox += mb * a.bf;
-        a.sc += mb;
-        if (a.m_inv_re) {
-            a.phys.inv_Re = a.m_inv_re[mb];
-            a.phys.Re = 1.0 / a.phys.inv_Re;
-        }
+    // member blockIdx.y of a batch: its fields, scalars and 1/Re (the
+    // argument block itself stays untouched, in constant kernarg memory)
+    const size_t mb = (MODE == COL_STAGE || MODE == COL_PRIME) ? blockIdx.y : 0;
+    const cx<T>* const nl = a.nl + mb * a.bm;
+    cx<T>* const state = a.state + mb * a.bc;
+    cx<T>* const acc = a.acc + mb * a.bc;
+    cx<T>* const stagebuf = a.stagebuf + mb * a.bc;
+    cx<T>* const sfd = a.sfd + mb * a.bc;
+    const cx<T>* const fbox = a.fbox + mb * a.bf;
+    DevScalars* const sc = a.sc + mb;
+    const size_t mixed_off = mb * a.bm;
+    Phys ph = a.phys;
+    if (a.m_inv_re) {
+        ph.inv_Re = a.m_inv_re[mb];
+        ph.Re = 1.0 / ph.inv_Re;
     }
     const Line g = make_line(LC > 0 ? LC : G.ny);
     const int TPL = LC > 0 ? LC / P : g.TPL;
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     // ---- forward y-transform of the mixed nonlinear field
     if constexpr (MODE == COL_STAGE || MODE == COL_FWD) {
 #pragma unroll
-        for (int s = 0; s < P; ++s) v[s] = has_data ? a.nl[mbase + j + s * TPL] : zero;
+        for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
         fft_any<-1, LC>(v, lds, j, g, a.tw);
     }
 
@@ -365,20 +366,20 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     // ---- spectral state of this column after the stage (or as is)
     double dt = 0.0;
     if constexpr (MODE == COL_STAGE) {
-        if (a.stage == 0 && a.phys.adaptive) {
-            const double u2 = __longlong_as_double((long long)a.sc->umax2_bits);
-            dt = cfl_dt(a.phys, sqrt(u2));
+        if (a.stage == 0 && ph.adaptive) {
+            const double u2 = __longlong_as_double((long long)sc->umax2_bits);
+            dt = cfl_dt(ph, sqrt(u2));
             if (blockIdx.x == 0 && threadIdx.x == 0) {
-                a.sc->dt = dt;
-                if (!(u2 <= 1.0e300)) a.sc->status = 2;  // SOLVER_DIVERGED
+                sc->dt = dt;
+                if (!(u2 <= 1.0e300)) sc->status = 2;  // SOLVER_DIVERGED
             }
         } else {
-            dt = a.sc->dt;
+            dt = sc->dt;
         }
     }
     const T h = T(dt);
     const T norm = T(1.0 / ((double)G.nx * (double)G.ny));
-    const double sfd_alpha = (a.phys.sfd_tau > 0.0) ? fmin(fmax(dt / a.phys.sfd_tau, 0.0), 1.0) : 0.0;
+    const double sfd_alpha = (ph.sfd_tau > 0.0) ? fmin(fmax(dt / ph.sfd_tau, 0.0), 1.0) : 0.0;
     const T sqh = T(sqrt(dt));
 
     // y-band of the 2/3 mask: compile-time for compile-time column lengths
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         const size_t ci = (size_t)kxi * ncy + (pos ? jy : jy - (Ll - ncy));
         const T ky = T(pos ? jy : jy - Ll) * T(2.0 * kPi / G.Ly);
         if constexpr (MODE == COL_PRIME) {
-            w[s] = a.state[ci];
+            w[s] = state[ci];
             continue;
         }
         if constexpr (MODE == COL_INV) {
@@ -404,72 +405,72 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         }
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
         const cx<T> N = (-norm) * v[s];
-        const cx<T> w0 = a.state[ci];
+        const cx<T> w0 = state[ci];
         const T k2 = kx * kx + ky * ky;
         const T kmag = sqrt(k2);
         cx<T> wn;
         if constexpr (SCHEME == 0) {
             // classic RK4 (spectral_time_integration.c:50-100) with
             // hyperviscosity_apply (hyperviscosity.c:104-150) and drag
-            const cx<T> ws = (a.stage == 0) ? w0 : a.stagebuf[ci];
-            const T lin = -(k2 * T(a.phys.inv_Re) + T(a.phys.drag) + hv_weight_t<T>(a.phys, kmag));
+            const cx<T> ws = (a.stage == 0) ? w0 : stagebuf[ci];
+            const T lin = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, kmag));
             cx<T> ks = N + lin * ws;
-            const T sw = sfd_weight_t<T>(a.phys, kmag);
-            if (sw > T(0)) ks = ks - sw * (ws - a.sfd[ci]);
+            const T sw = sfd_weight_t<T>(ph, kmag);
+            if (sw > T(0)) ks = ks - sw * (ws - sfd[ci]);
             if (a.stage == 0) {
-                a.acc[ci] = w0 + (h / T(6)) * ks;
+                acc[ci] = w0 + (h / T(6)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
             } else if (a.stage == 1) {
-                a.acc[ci] = a.acc[ci] + (h / T(3)) * ks;
+                acc[ci] = acc[ci] + (h / T(3)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
             } else if (a.stage == 2) {
-                a.acc[ci] = a.acc[ci] + (h / T(3)) * ks;
+                acc[ci] = acc[ci] + (h / T(3)) * ks;
                 wn = w0 + h * ks;
             } else {
-                wn = a.acc[ci] + (h / T(6)) * ks;
-                if (a.phys.sfd_on && a.phys.sfd_tau > 0.0) {  // hyperviscosity.c:152-178
+                wn = acc[ci] + (h / T(6)) * ks;
+                if (ph.sfd_on && ph.sfd_tau > 0.0) {  // hyperviscosity.c:152-178
                     const T al = T(sfd_alpha);
-                    a.sfd[ci] = (T(1) - al) * a.sfd[ci] + al * wn;
+                    sfd[ci] = (T(1) - al) * sfd[ci] + al * wn;
                 }
             }
-            if (a.stage < 3) a.stagebuf[ci] = wn;
+            if (a.stage < 3) stagebuf[ci] = wn;
         } else {
             // integrating factor, L = -(k^2/Re + w_hv + mu).  The factors are
             // carried as E - 1 (expm1): near k = 0 the per-step decay L h is
             // ~1e-5, which 1 + (E - 1) keeps to full relative precision while
             // E itself, rounded to FP32, would lose ~0.3 % of it every step.
-            const T L = -(k2 * T(a.phys.inv_Re) + T(a.phys.drag) + hv_weight_t<T>(a.phys, kmag));
+            const T L = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, kmag));
             const T e2 = expm1(L * (T(0.5) * h));  // E2 - 1, E2 = exp(L h / 2)
             const T e1 = e2 * (T(2) + e2);          // E - 1,  E = E2^2
             if (a.stage == 0) {
-                a.acc[ci] = grow(w0, e1) + (h / T(6)) * grow(N, e1);
+                acc[ci] = grow(w0, e1) + (h / T(6)) * grow(N, e1);
                 wn = grow(w0 + (T(0.5) * h) * N, e2);
             } else if (a.stage == 1) {
-                a.acc[ci] = a.acc[ci] + (h / T(3)) * grow(N, e2);
+                acc[ci] = acc[ci] + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e2) + (T(0.5) * h) * N;
             } else if (a.stage == 2) {
-                a.acc[ci] = a.acc[ci] + (h / T(3)) * grow(N, e2);
+                acc[ci] = acc[ci] + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e1) + h * grow(N, e2);
             } else {
-                wn = a.acc[ci] + (h / T(6)) * N;
+                wn = acc[ci] + (h / T(6)) * N;
             }
         }
         if (a.stage == 3) {
-            if (a.phys.force_on && kxi < a.phys.fbx) {
+            if (ph.force_on && kxi < ph.fbx) {
                 const int b = band_of(G, jy);
-                if (b >= -a.phys.fby && b <= a.phys.fby)
-                    wn = wn + sqh * a.fbox[kxi * (2 * a.phys.fby + 1) + (b + a.phys.fby)];
+                if (b >= -ph.fby && b <= ph.fby)
+                    wn = wn + sqh * fbox[kxi * (2 * ph.fby + 1) + (b + ph.fby)];
             }
-            a.state[ci] = wn;
+            state[ci] = wn;
         }
         w[s] = wn;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (MODE == COL_STAGE && a.stage == 3) {
-            a.sc->t += dt;
-            a.sc->umax2_bits = 0ull;
+            sc->t += dt;
+            sc->umax2_bits = 0ull;
         } else if (MODE == COL_PRIME) {
-            a.sc->umax2_bits = 0ull;
+            sc->umax2_bits = 0ull;
         }
     }
 
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
             fft_any<+1, LC>(v, lds, j, g, a.tw);
             if (has_data) {
-                cx<T>* out = a.mixed[f];
+                cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
                 for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
@@ -517,8 +518,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
             line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);
             if (has_data) {
-                cx<T>* o1 = a.mixed[f];
-                cx<T>* o2 = a.mixed[f - 1];
+                cx<T>* o1 = a.mixed[f] + mixed_off;
+                cx<T>* o2 = a.mixed[f - 1] + mixed_off;
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
                     o1[mbase + j + s * TPL] = v[s];
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
             fft_any<+1, LC>(v, lds, j, g, a.tw);
             if (has_data) {
-                cx<T>* out = a.mixed[f];
+                cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
                 for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
@@ -719,12 +720,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     cx<T> v[P];
-    if constexpr (MODE == ROW_STAGE) {
-        const size_t mb = blockIdx.y;
-        for (int i = 0; i < 4; ++i) a.mixed[i] += mb * a.bm;
-        a.nl += mb * a.bm;
-        a.sc += mb;
-    }
+    // member blockIdx.y of a batch (ROW_STAGE only)
+    const size_t moff = (MODE == ROW_STAGE) ? blockIdx.y * a.bm : 0;
 
     if constexpr (MODE == ROW_C2R) {
         const int y = item;
@@ -772,7 +769,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     T u2max = T(0);
     if constexpr (paired_lines<T, LC>()) {
         cx<T> v2[P];
-        build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0], a.mixed[1], y0, active);
+        build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0] + moff, a.mixed[1] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y
 #pragma unroll
         for (int s = 0; s < P; ++s) {
@@ -785,20 +782,20 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
             X[s] = v[s];
             Jp[s] = v2[s];
         }
-        build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[2], a.mixed[3], y0, active);
+        build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[2] + moff, a.mixed[3] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // omega_x, omega_y
 #pragma unroll
         for (int s = 0; s < P; ++s)
             v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
-        fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
-        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc->umax2_bits, red);
+        fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
+        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
-    build_two_rows<LC>(v, lds, j, g, G, a.mixed[0], y0, active);  // psi_x
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[0] + moff, y0, active);  // psi_x
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) X[s] = v[s];
-    build_two_rows<LC>(v, lds, j, g, G, a.mixed[1], y0, active);  // psi_y
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[1] + moff, y0, active);  // psi_y
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) {
@@ -811,17 +808,17 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         u2max = sb > u2max ? sb : u2max;
         Jp[s] = v[s];  // psi_y, folded with omega_x next
     }
-    build_two_rows<LC>(v, lds, j, g, G, a.mixed[2], y0, active);  // omega_x
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[2] + moff, y0, active);  // omega_x
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
-    build_two_rows<LC>(v, lds, j, g, G, a.mixed[3], y0, active);  // omega_y
+    build_two_rows<LC>(v, lds, j, g, G, a.mixed[3] + moff, y0, active);  // omega_y
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
     for (int s = 0; s < P; ++s)
         v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
-    fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
-    if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc->umax2_bits, red);
+    fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
+    if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
 }
 
 }  // namespace nsd
