@@ -107,6 +107,16 @@ int nsd_batch_scalars(NsdContext *c, double *t, double *dt, int *status);
 int nsd_member_upload(NsdContext *c, int member, const double *omega_hat);
 int nsd_member_download(NsdContext *c, int member, double *omega_hat);
 
+/* Launch variants of the two time-step kernels (workgroup size and whether
+ * inverse transforms run in lock-step pairs; lengths 64..1024 have several).
+ * nsd_autotune times every variant on this context's shape (reps launches
+ * each) and keeps the fastest; results are cached per (length, precision)
+ * for the process and, if the environment names a file in NSD_TUNE_FILE,
+ * across processes (force != 0 re-times).  It zeroes the state: call it
+ * before setting the initial field.  Every variant gives identical results. */
+int nsd_autotune(NsdContext *c, int reps, int force, int *row_variant, int *col_variant);
+int nsd_set_variants(NsdContext *c, int row_variant, int col_variant);
+
 /* One captured hipGraph that advances several batch contexts (different grid
  * sizes, same precision) by one step per launch; each context's kernels run
  * on its own stream inside the graph, so the sizes overlap on the device.

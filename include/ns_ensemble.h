@@ -31,6 +31,7 @@ typedef struct {
     double injection_rate;      /* energy injection rate of the forcing               */
     double k_peak;              /* random-phase initial field: E(k) ~ k^4 e^-(k/k_peak)^2 */
     double energy;              /* ... normalised to this energy                       */
+    int autotune;               /* choose kernel launch variants per size, default 1  */
 } NSEnsembleConfig;
 
 typedef struct NSEnsemble NSEnsemble;

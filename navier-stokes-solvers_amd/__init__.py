@@ -67,7 +67,7 @@ EXPORTED = [
     "nsd_create_batch", "nsd_batch_members", "nsd_set_member_params", "nsd_batch_random_phase",
     "nsd_batch_energy", "nsd_batch_scalars", "nsd_member_upload", "nsd_member_download",
     "nsd_multi_create", "nsd_multi_destroy", "nsd_multi_error", "nsd_multi_advance",
-    "nsd_multi_synchronize",
+    "nsd_multi_synchronize", "nsd_autotune", "nsd_set_variants",
     # ns_ensemble.h
     "ns_ensemble_config_default", "ns_ensemble_create", "ns_ensemble_create_error",
     "ns_ensemble_free", "ns_ensemble_members", "ns_ensemble_advance", "ns_ensemble_synchronize",
@@ -143,7 +143,8 @@ class NSEnsembleConfig(C.Structure):  # ns_ensemble.h
     _fields_ = [("Lx", C.c_double), ("Ly", C.c_double), ("precision", C.c_int),
                 ("scheme", C.c_int), ("cfl_factor", C.c_double), ("drag", C.c_double),
                 ("nu4", C.c_double), ("forcing_enabled", C.c_int), ("forcing", NSForcingConfig),
-                ("injection_rate", C.c_double), ("k_peak", C.c_double), ("energy", C.c_double)]
+                ("injection_rate", C.c_double), ("k_peak", C.c_double), ("energy", C.c_double),
+                ("autotune", C.c_int)]
 
 
 _lib = None
@@ -301,6 +302,8 @@ def load():
         "nsd_multi_error": ([V], C.c_char_p),
         "nsd_multi_advance": ([V, C.c_int, C.c_int], C.c_int),
         "nsd_multi_synchronize": ([V], C.c_int),
+        "nsd_autotune": ([V, C.c_int, C.c_int, I, I], C.c_int),
+        "nsd_set_variants": ([V, C.c_int, C.c_int], C.c_int),
         "ns_ensemble_config_default": ([C.POINTER(NSEnsembleConfig)], None),
         "ns_ensemble_create": ([C.c_int, I, D, ULLP, C.POINTER(NSEnsembleConfig)], V),
         "ns_ensemble_create_error": ([], C.c_char_p),

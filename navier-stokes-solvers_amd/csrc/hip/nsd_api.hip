@@ -5,8 +5,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "ns_device.h"
@@ -554,6 +558,8 @@ struct NsdContext {
     Geo G;
     int prec;
     int nb = 1;                             // batch members (one field each)
+    int vrow = 0, vcol = 0;                 // launch variants of the step kernels
+    int maxlds = 65536;                     // LDS bytes per workgroup on this device
     double* m_inv_re = nullptr;             // per-member 1/Re (batches), device
     unsigned long long* m_seed = nullptr;   // per-member forcing seeds (batches), device
     unsigned long long* m_ic = nullptr;     // per-member initial-field seeds, device
@@ -682,7 +688,7 @@ template <typename T>
 void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
     const int L = c->G.ny;
     if (a.mode == COL_STAGE || a.mode == COL_PRIME) {
-        launch_col_fused<T>(a, L, ncols, c->nb, c->stream);
+        launch_col_fused<T>(a, L, ncols, c->nb, c->vcol, c->stream);
         return;
     }
     const int NT = threads_for(L);
@@ -706,7 +712,7 @@ template <typename T>
 void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items) {
     const int L = c->G.nx;
     if (a.mode == ROW_STAGE) {
-        launch_row_stage<T>(a, L, items, c->nb, c->stream);
+        launch_row_stage<T>(a, L, items, c->nb, c->vrow, c->stream);
         return;
     }
     const int NT = threads_for(L);
@@ -1021,6 +1027,114 @@ int restore_mixed(NsdContext* c) {
     return op_prime<T>(c);
 }
 
+// ----------------------------------------------------------------- autotuner
+
+// Tuned launch variants per (transform length, precision, kernel): timed once
+// per process, optionally persisted in the text file named by NSD_TUNE_FILE
+// (lines "row|col L precision variant").
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune;
+bool g_tune_loaded = false;
+
+void tune_load() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char* f = getenv("NSD_TUNE_FILE");
+    if (!f) return;
+    FILE* fp = fopen(f, "r");
+    if (!fp) return;
+    char kind[8];
+    int L, prec, v;
+    while (fscanf(fp, "%7s %d %d %d", kind, &L, &prec, &v) == 4)
+        g_tune[std::make_tuple(strcmp(kind, "col") == 0 ? 1 : 0, L, prec)] = v;
+    fclose(fp);
+}
+
+void tune_store(int kind, int L, int prec, int v) {
+    g_tune[std::make_tuple(kind, L, prec)] = v;
+    const char* f = getenv("NSD_TUNE_FILE");
+    if (!f) return;
+    FILE* fp = fopen(f, "a");
+    if (!fp) return;
+    fprintf(fp, "%s %d %d %d\n", kind ? "col" : "row", L, prec, v);
+    fclose(fp);
+}
+
+// zero every field and scalar the step reads, so that timing runs leave a
+// well-defined (all-zero) state behind
+int zero_fields(NsdContext* c) {
+    const size_t C = C_of(c) * c->esz * c->nb, M = M_of(c) * c->esz * c->nb;
+    for (void* b : {c->state, c->acc, c->stagebuf, c->sfd})
+        CK(hipMemsetAsync(b, 0, C, c->stream));
+    for (void* b : {c->mixed[0], c->mixed[1], c->mixed[2], c->mixed[3], c->nl})
+        CK(hipMemsetAsync(b, 0, M, c->stream));
+    CK(hipMemsetAsync(c->sc, 0, sizeof(DevScalars) * c->nb, c->stream));
+    return sync(c);
+}
+
+template <typename T>
+int op_autotune(NsdContext* c, int reps, int force) {
+    if (zero_fields(c)) return ST_MEM;
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    const int prec = c->prec;
+    for (int kind = 0; kind < 2; ++kind) {
+        const int L = kind == 0 ? c->G.nx : c->G.ny;
+        int best = 0;
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            tune_load();
+            auto it = g_tune.find(std::make_tuple(kind, L, prec));
+            if (it != g_tune.end() && !force) {
+                best = it->second;
+                cached = true;
+            }
+        }
+        if (!cached) {
+            float best_ms = 1e30f;
+            for (int v = 0; v < kStageVariants; ++v) {
+                if (!stage_variant_valid<T>(L, v, c->maxlds)) continue;
+                auto launch = [&]() {
+                    if (kind == 0) {
+                        RowArgs<T> r = row_args<T>(c);
+                        r.mode = ROW_STAGE;
+                        r.track_umax = 0;
+                        launch_row_stage<T>(r, L, c->G.ny / 2, c->nb, v, c->stream);
+                    } else {
+                        ColArgs<T> a = col_args<T>(c);
+                        a.mode = COL_STAGE;
+                        a.stage = 1;
+                        a.phys.adaptive = 0;
+                        launch_col_fused<T>(a, L, c->G.nkx_keep, c->nb, v, c->stream);
+                    }
+                };
+                launch();  // warm-up
+                CK(hipEventRecord(e0, c->stream));
+                for (int r = 0; r < reps; ++r) launch();
+                CK(hipEventRecord(e1, c->stream));
+                CK(hipEventSynchronize(e1));
+                if (lastk(c)) return ST_MEM;
+                float ms = 0.f;
+                CK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = v;
+                }
+            }
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            tune_store(kind, L, prec, best);
+        }
+        if (kind == 0) c->vrow = best;
+        else c->vcol = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->gen++;
+    return zero_fields(c);
+}
+
 // ----------------------------------------------------------------- batches
 
 int batch_refused(NsdContext* c) {
@@ -1229,6 +1343,7 @@ NsdContext* nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision
     }
     if (precision == NSD_FP32) set_lds_limits<float>(maxlds);
     else set_lds_limits<double>(maxlds);
+    c->maxlds = maxlds;
     nsd_set_params(c, &c->P);
     return c;
 }
@@ -1427,6 +1542,31 @@ int nsd_state_energy(NsdContext* c, double out[2]) {
 }
 
 int nsd_batch_members(const NsdContext* c) { return c ? c->nb : 0; }
+
+int nsd_autotune(NsdContext* c, int reps, int force, int* row_variant, int* col_variant) {
+    if (!c || reps < 1) return ST_PARAM;
+    const int rc = TYPED(op_autotune, c, reps, force);
+    if (row_variant) *row_variant = c->vrow;
+    if (col_variant) *col_variant = c->vcol;
+    return rc;
+}
+
+int nsd_set_variants(NsdContext* c, int row_variant, int col_variant) {
+    if (!c) return ST_PARAM;
+    const bool ok = c->prec == NSD_FP32
+                        ? stage_variant_valid<float>(c->G.nx, row_variant, c->maxlds) &&
+                              stage_variant_valid<float>(c->G.ny, col_variant, c->maxlds)
+                        : stage_variant_valid<double>(c->G.nx, row_variant, c->maxlds) &&
+                              stage_variant_valid<double>(c->G.ny, col_variant, c->maxlds);
+    if (!ok) {
+        c->err = "launch variant not available for this size";
+        return ST_PARAM;
+    }
+    c->vrow = row_variant;
+    c->vcol = col_variant;
+    c->gen++;
+    return ST_OK;
+}
 
 int nsd_set_member_params(NsdContext* c, const double* Re, const unsigned long long* force_seed) {
     if (!c) return ST_PARAM;

@@ -210,16 +210,18 @@ struct Band {
 };
 
 // Whether the time-step kernels run their inverse transforms two at a time
-// (needs two line regions per line in LDS; kept within 80 KB per workgroup
-// so that two workgroups share a CU).
-template <typename T, int LC>
+// (needs two line regions per line in LDS).  PR < 0 decides automatically:
+// paired when the workgroup's two regions stay within 80 KB, so that two
+// workgroups share a CU; PR = 0 / 1 forces the choice (launch variants).
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
 constexpr bool paired_lines() {
-    return LC > 0 && 2 * (LC + LC / 16) * (int)sizeof(cx<T>) * (threads_for(LC) * P / LC) <=
-                         80 * 1024;
+    if (LC <= 0) return false;
+    if (PR >= 0) return PR == 1;
+    return 2 * (LC + LC / 16) * (int)sizeof(cx<T>) * (NT * P / LC) <= 80 * 1024;
 }
-template <typename T, int LC>
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
 constexpr int line_regions() {
-    return paired_lines<T, LC>() ? 2 : 1;
+    return paired_lines<T, LC, NT, PR>() ? 2 : 1;
 }
 
 constexpr int slot_kind(int s, int L, int km) {
@@ -304,7 +306,7 @@ __device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
 template <typename T>
 constexpr int min_waves() { return sizeof(T) == 4 ? 4 : 2; }
 
-template <typename T, int NT, int MODE, int SCHEME, int LC>
+template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    constexpr int NB = line_regions<T, LC>();
+    constexpr int NB = line_regions<T, LC, NT, PR>();
     cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int kxi = blockIdx.x * lines + m;
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
-    } else if constexpr (paired_lines<T, LC>()) {
+    } else if constexpr (paired_lines<T, LC, NT, PR>()) {
         // outputs 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99),
         // transformed two at a time: (omega_y, omega_x), then w becomes
         // psi_hat = -w/k^2 in place and (psi_y, psi_x)
@@ -703,7 +705,7 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
     }
 }
 
-template <typename T, int NT, int MODE, int LC>
+template <typename T, int NT, int MODE, int LC, int PR = -1>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
@@ -714,7 +716,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    constexpr int NB = line_regions<T, LC>();
+    constexpr int NB = line_regions<T, LC, NT, PR>();
     cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -767,7 +769,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
     T u2max = T(0);
-    if constexpr (paired_lines<T, LC>()) {
+    if constexpr (paired_lines<T, LC, NT, PR>()) {
         cx<T> v2[P];
         build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0] + moff, a.mixed[1] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y

@@ -1,6 +1,10 @@
 // nsd_stage.hip -- time-step kernels instantiated for every supported
 // transform length, so that all LDS addresses and strides are compile-time
 // constants.  Compiled once per precision with -DNSD_T=float / double.
+//
+// Lengths 64..1024 (the ensemble sizes) also get launch variants that the
+// autotuner chooses between: workgroup size (lines per workgroup) and whether
+// the inverse transforms run in lock-step pairs.
 #include "nsd_stage.hpp"
 
 #ifndef NSD_T
@@ -16,32 +20,60 @@ constexpr int nt_for() {
     return L / P > 256 ? L / P : 256;
 }
 
-template <typename T, int LOG>
-void row_stage_L(const RowArgs<T>& a, int items, int members, hipStream_t st) {
-    constexpr int L = 1 << LOG;
-    constexpr int NT = nt_for<L>();
-    constexpr int lines = NT * P / L;
-    const int blocks = (items + lines - 1) / lines;
-    const size_t lds = (size_t)lines * line_regions<T, L>() * (L + L / 16) * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L>), dim3(blocks, members), dim3(NT), lds, st, a);
+// variant V of length L: threads per workgroup and pairing (-1 automatic)
+template <int L, int V>
+constexpr int var_nt() {
+    return V <= 1 ? nt_for<L>() : (V <= 3 ? 2 * nt_for<L>() : nt_for<L>() / 2);
+}
+template <int V>
+constexpr int var_pr() {
+    return (V == 1 || V == 3) ? 0 : -1;
+}
+template <int L, int V>
+constexpr bool var_ok() {
+    constexpr int NT = var_nt<L, V>();
+    return V == 0 || (L >= 64 && L <= 1024 && NT <= 1024 && NT >= L / P && NT >= 64);
 }
 
-template <typename T, int LOG>
-void col_fused_L(const ColArgs<T>& a, int ncols, int members, hipStream_t st) {
-    constexpr int L = 1 << LOG;
-    constexpr int NT = nt_for<L>();
+template <typename T, int L, int V>
+size_t var_lds() {
+    constexpr int NT = var_nt<L, V>();
     constexpr int lines = NT * P / L;
-    const int blocks = (ncols + lines - 1) / lines;
-    const size_t lds = (size_t)lines * line_regions<T, L>() * (L + L / 16) * sizeof(cx<T>);
-    if (a.mode == COL_PRIME)
-        hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L>), dim3(blocks, members), dim3(NT), lds, st,
-                           a);
-    else if (a.phys.scheme == 0)
-        hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 0, L>), dim3(blocks, members), dim3(NT), lds,
+    constexpr int PR = var_pr<V>();
+    return (size_t)lines * line_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
+}
+
+template <typename T, int LOG, int V>
+void row_stage_LV(const RowArgs<T>& a, int items, int members, hipStream_t st) {
+    constexpr int L = 1 << LOG;
+    if constexpr (var_ok<L, V>()) {
+        constexpr int NT = var_nt<L, V>();
+        constexpr int PR = var_pr<V>();
+        constexpr int lines = NT * P / L;
+        const int blocks = (items + lines - 1) / lines;
+        const size_t lds = var_lds<T, L, V>();
+        hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L, PR>), dim3(blocks, members), dim3(NT), lds,
                            st, a);
-    else
-        hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 1, L>), dim3(blocks, members), dim3(NT), lds,
-                           st, a);
+    }
+}
+
+template <typename T, int LOG, int V>
+void col_fused_LV(const ColArgs<T>& a, int ncols, int members, hipStream_t st) {
+    constexpr int L = 1 << LOG;
+    if constexpr (var_ok<L, V>()) {
+        constexpr int NT = var_nt<L, V>();
+        constexpr int PR = var_pr<V>();
+        constexpr int lines = NT * P / L;
+        const int blocks = (ncols + lines - 1) / lines;
+        const dim3 grid(blocks, members);
+        const size_t lds = var_lds<T, L, V>();
+        if (a.mode == COL_PRIME)
+            hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L, PR>), grid, dim3(NT), lds, st, a);
+        else if (a.phys.scheme == 0)
+            hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 0, L, PR>), grid, dim3(NT), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 1, L, PR>), grid, dim3(NT), lds, st, a);
+    }
 }
 
 template <typename K>
@@ -50,14 +82,24 @@ void lds_attr(K kernel, int bytes) {
                               bytes);
 }
 
-template <typename T, int LOG>
-void limits_L(int bytes) {
+template <typename T, int LOG, int V>
+void limits_LV(int bytes) {
     constexpr int L = 1 << LOG;
-    constexpr int NT = nt_for<L>();
-    lds_attr(k_row<T, NT, ROW_STAGE, L>, bytes);
-    lds_attr(k_col<T, NT, COL_PRIME, 0, L>, bytes);
-    lds_attr(k_col<T, NT, COL_STAGE, 0, L>, bytes);
-    lds_attr(k_col<T, NT, COL_STAGE, 1, L>, bytes);
+    if constexpr (var_ok<L, V>()) {
+        constexpr int NT = var_nt<L, V>();
+        constexpr int PR = var_pr<V>();
+        lds_attr(k_row<T, NT, ROW_STAGE, L, PR>, bytes);
+        lds_attr(k_col<T, NT, COL_PRIME, 0, L, PR>, bytes);
+        lds_attr(k_col<T, NT, COL_STAGE, 0, L, PR>, bytes);
+        lds_attr(k_col<T, NT, COL_STAGE, 1, L, PR>, bytes);
+    }
+}
+
+template <typename T, int LOG, int V>
+int var_valid(int maxlds) {
+    constexpr int L = 1 << LOG;
+    if constexpr (var_ok<L, V>()) return var_lds<T, L, V>() <= (size_t)maxlds ? 1 : 0;
+    return 0;
 }
 
 int log2i(int L) {
@@ -69,38 +111,56 @@ int log2i(int L) {
 }  // namespace
 
 #define NSD_FOR_LOGS(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
+#define NSD_FOR_VARS(X, n) X(n, 0) X(n, 1) X(n, 2) X(n, 3) X(n, 4)
 
 template <typename T>
-void launch_row_stage(const RowArgs<T>& a, int L, int items, int members, hipStream_t st) {
-    switch (log2i(L)) {
-#define X(n) \
-    case n: row_stage_L<T, n>(a, items, members, st); break;
-        NSD_FOR_LOGS(X)
+void launch_row_stage(const RowArgs<T>& a, int L, int items, int members, int variant,
+                      hipStream_t st) {
+    const int lg = log2i(L);
+#define V_(n, v) \
+    if (lg == n && variant == v) return row_stage_LV<T, n, v>(a, items, members, st);
+#define X(n) NSD_FOR_VARS(V_, n)
+    NSD_FOR_LOGS(X)
 #undef X
-    default: break;
-    }
+#undef V_
 }
 
 template <typename T>
-void launch_col_fused(const ColArgs<T>& a, int L, int ncols, int members, hipStream_t st) {
-    switch (log2i(L)) {
-#define X(n) \
-    case n: col_fused_L<T, n>(a, ncols, members, st); break;
-        NSD_FOR_LOGS(X)
+void launch_col_fused(const ColArgs<T>& a, int L, int ncols, int members, int variant,
+                      hipStream_t st) {
+    const int lg = log2i(L);
+#define V_(n, v) \
+    if (lg == n && variant == v) return col_fused_LV<T, n, v>(a, ncols, members, st);
+#define X(n) NSD_FOR_VARS(V_, n)
+    NSD_FOR_LOGS(X)
 #undef X
-    default: break;
-    }
+#undef V_
+}
+
+template <typename T>
+int stage_variant_valid(int L, int variant, int maxlds) {
+    const int lg = log2i(L);
+#define V_(n, v) \
+    if (lg == n && variant == v) return var_valid<T, n, v>(maxlds);
+#define X(n) NSD_FOR_VARS(V_, n)
+    NSD_FOR_LOGS(X)
+#undef X
+#undef V_
+    return 0;
 }
 
 template <typename T>
 void stage_lds_limits(int bytes) {
-#define X(n) limits_L<T, n>(bytes);
+#define V_(n, v) limits_LV<T, n, v>(bytes);
+#define X(n) NSD_FOR_VARS(V_, n)
     NSD_FOR_LOGS(X)
 #undef X
+#undef V_
 }
 
-template void launch_row_stage<NSD_T>(const RowArgs<NSD_T>&, int, int, int, hipStream_t);
-template void launch_col_fused<NSD_T>(const ColArgs<NSD_T>&, int, int, int, hipStream_t);
+template void launch_row_stage<NSD_T>(const RowArgs<NSD_T>&, int, int, int, int, hipStream_t);
+template void launch_col_fused<NSD_T>(const ColArgs<NSD_T>&, int, int, int, int, hipStream_t);
+template int stage_variant_valid<NSD_T>(int, int, int);
 template void stage_lds_limits<NSD_T>(int);
 
 }  // namespace nsd

@@ -41,6 +41,7 @@ void ns_ensemble_config_default(NSEnsembleConfig *cfg) {
     ns_forcing_config_default(&cfg->forcing);
     cfg->k_peak = 10.0;
     cfg->energy = 1.0;
+    cfg->autotune = 1;
 }
 
 static void params_for(const NSEnsembleConfig *cfg, int n, NsdParams *p) {
@@ -145,7 +146,9 @@ NSEnsemble *ns_ensemble_create(int members, const int *n, const double *Re,
             re[s] = Re[G->member[s]];
             sd[s] = seeds[G->member[s]];
         }
-        if (nsd_set_params(G->ctx, &p) || nsd_set_member_params(G->ctx, re, sd) ||
+        if (nsd_set_params(G->ctx, &p) ||
+            (cfg.autotune && nsd_autotune(G->ctx, 3, 0, NULL, NULL)) ||
+            nsd_set_member_params(G->ctx, re, sd) ||
             nsd_batch_random_phase(G->ctx, sd, cfg.k_peak, cfg.energy) ||
             nsd_set_time(G->ctx, 0.0, 0.0))
             why = nsd_error(G->ctx);
