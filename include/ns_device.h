@@ -66,6 +66,11 @@ int nsd_set_time(NsdContext *c, double t, double dt);
 int nsd_get_time(NsdContext *c, double *t, double *dt, int *status);
 int nsd_set_forcing_counter(NsdContext *c, unsigned long long calls);
 int nsd_get_forcing_counter(NsdContext *c, unsigned long long *calls);
+/* coloured-forcing (OU) state of member 0: 2 * nsd_forcing_modes() doubles,
+ * interleaved complex in the active-mode order of forcing.c:80-93;
+ * to_device != 0 uploads, else downloads */
+int nsd_forcing_ou(NsdContext *c, double *ou, int count, int to_device);
+int nsd_forcing_modes(const NsdContext *c);
 int nsd_advance(NsdContext *c, int nsteps, int adaptive);
 int nsd_advance_graph(NsdContext *c, int nsteps, int adaptive);
 int nsd_state_derived(NsdContext *c, double *psi_hat, double *u_hat, double *v_hat, double *u,

@@ -67,7 +67,10 @@ EXPORTED = [
     "nsd_create_batch", "nsd_batch_members", "nsd_set_member_params", "nsd_batch_random_phase",
     "nsd_batch_energy", "nsd_batch_scalars", "nsd_member_upload", "nsd_member_download",
     "nsd_multi_create", "nsd_multi_destroy", "nsd_multi_error", "nsd_multi_advance",
-    "nsd_multi_synchronize", "nsd_autotune", "nsd_set_variants",
+    "nsd_multi_synchronize", "nsd_autotune", "nsd_set_variants", "nsd_forcing_ou",
+    "nsd_forcing_modes",
+    # checkpoint.h
+    "ns_checkpoint_write_spectral", "ns_checkpoint_read_spectral", "ns_checkpoint_error",
     # ns_ensemble.h
     "ns_ensemble_config_default", "ns_ensemble_create", "ns_ensemble_create_error",
     "ns_ensemble_free", "ns_ensemble_members", "ns_ensemble_advance", "ns_ensemble_synchronize",
@@ -304,6 +307,11 @@ def load():
         "nsd_multi_synchronize": ([V], C.c_int),
         "nsd_autotune": ([V, C.c_int, C.c_int, I, I], C.c_int),
         "nsd_set_variants": ([V, C.c_int, C.c_int], C.c_int),
+        "nsd_forcing_ou": ([V, D, C.c_int, C.c_int], C.c_int),
+        "nsd_forcing_modes": ([V], C.c_int),
+        "ns_checkpoint_write_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
+        "ns_checkpoint_read_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
+        "ns_checkpoint_error": ([], C.c_char_p),
         "ns_ensemble_config_default": ([C.POINTER(NSEnsembleConfig)], None),
         "ns_ensemble_create": ([C.c_int, I, D, ULLP, C.POINTER(NSEnsembleConfig)], V),
         "ns_ensemble_create_error": ([], C.c_char_p),

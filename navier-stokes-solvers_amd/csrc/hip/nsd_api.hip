@@ -1514,6 +1514,21 @@ int nsd_get_forcing_counter(NsdContext* c, unsigned long long* calls) {
     return sync(c);
 }
 
+int nsd_forcing_ou(NsdContext* c, double* ou, int count, int to_device) {
+    if (!c) return ST_PARAM;
+    if (count != 2 * c->fA || (count > 0 && !ou)) {
+        c->err = "forcing OU state: count must be twice the number of active modes";
+        return ST_PARAM;
+    }
+    if (count == 0) return ST_OK;
+    const size_t b = sizeof(double) * (size_t)count;
+    if (to_device) CK(hipMemcpyAsync(c->fou, ou, b, hipMemcpyHostToDevice, c->stream));
+    else CK(hipMemcpyAsync(ou, c->fou, b, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int nsd_forcing_modes(const NsdContext* c) { return c ? c->fA : 0; }
+
 int nsd_advance(NsdContext* c, int nsteps, int adaptive) {
     if (!c || nsteps < 0) return ST_PARAM;
     if (nsteps == 0) return ST_OK;

@@ -11,20 +11,7 @@
 #include <stdint.h>
 
 #include "ns_device.h"
-
-struct NSSpectralDevice {
-    NsdContext *ctx;
-    int precision;
-    int scheme;
-    double drag;
-    int force_on;
-    NSForcingConfig force_cfg;
-    double force_rate;
-    SolverStatus status;
-    char err[256];
-    NsdParams applied;
-    int applied_valid;
-};
+#include "spectral_internal.h"
 
 static struct ns_fft_plan_s { int unused; } ns_plan_token;
 
