@@ -15,6 +15,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnsspectral.so")
+DRIVER_PATH = os.path.join(_HERE, "ns_spectral_run")
 
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
@@ -71,6 +72,8 @@ EXPORTED = [
     "nsd_forcing_modes",
     # checkpoint.h
     "ns_checkpoint_write_spectral", "ns_checkpoint_read_spectral", "ns_checkpoint_error",
+    # vtk_writer.h
+    "ns_vtk_write_spectral_legacy", "ns_vtk_write_spectral_xml", "ns_vtk_update_pvd",
     # ns_ensemble.h
     "ns_ensemble_config_default", "ns_ensemble_create", "ns_ensemble_create_error",
     "ns_ensemble_free", "ns_ensemble_members", "ns_ensemble_advance", "ns_ensemble_synchronize",
@@ -312,6 +315,11 @@ def load():
         "ns_checkpoint_write_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
         "ns_checkpoint_read_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
         "ns_checkpoint_error": ([], C.c_char_p),
+        "ns_vtk_write_spectral_legacy": ([C.c_char_p, C.POINTER(NSSpectralData), C.c_double],
+                                         C.c_int),
+        "ns_vtk_write_spectral_xml": ([C.c_char_p, C.POINTER(NSSpectralData), C.c_double],
+                                      C.c_int),
+        "ns_vtk_update_pvd": ([C.c_char_p, C.c_char_p, C.c_double], C.c_int),
         "ns_ensemble_config_default": ([C.POINTER(NSEnsembleConfig)], None),
         "ns_ensemble_create": ([C.c_int, I, D, ULLP, C.POINTER(NSEnsembleConfig)], V),
         "ns_ensemble_create_error": ([], C.c_char_p),
