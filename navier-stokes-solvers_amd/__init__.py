@@ -72,6 +72,9 @@ EXPORTED = [
     "nsd_forcing_modes",
     # checkpoint.h
     "ns_checkpoint_write_spectral", "ns_checkpoint_read_spectral", "ns_checkpoint_error",
+    # ns_closure.h
+    "nsd_closure_create", "nsd_closure_destroy", "nsd_closure_error", "nsd_closure_param_count",
+    "nsd_closure_params", "nsd_closure_forward", "nsd_closure_gradient",
     # vtk_writer.h
     "ns_vtk_write_spectral_legacy", "ns_vtk_write_spectral_xml", "ns_vtk_update_pvd",
     # ns_ensemble.h
@@ -320,6 +323,15 @@ def load():
         "ns_vtk_write_spectral_xml": ([C.c_char_p, C.POINTER(NSSpectralData), C.c_double],
                                       C.c_int),
         "ns_vtk_update_pvd": ([C.c_char_p, C.c_char_p, C.c_double], C.c_int),
+        "nsd_closure_create": ([C.c_int, C.c_int, ull, C.c_char_p, C.c_size_t], V),
+        "nsd_closure_destroy": ([V], None),
+        "nsd_closure_error": ([V], C.c_char_p),
+        "nsd_closure_param_count": ([V], C.c_int),
+        "nsd_closure_params": ([V, C.POINTER(C.c_float), C.c_int], C.c_int),
+        "nsd_closure_forward": ([V, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)],
+                                C.c_int),
+        "nsd_closure_gradient": ([V, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                  D, C.POINTER(C.c_float)], C.c_int),
         "ns_ensemble_config_default": ([C.POINTER(NSEnsembleConfig)], None),
         "ns_ensemble_create": ([C.c_int, I, D, ULLP, C.POINTER(NSEnsembleConfig)], V),
         "ns_ensemble_create_error": ([], C.c_char_p),

@@ -1,0 +1,532 @@
+// nsd_closure.hip -- the learned subgrid closure: a 4-layer 3x3 CNN
+// (1 -> 32 -> 32 -> 32 -> 1 channels, GELU, periodic padding) evaluated and
+// trained with bf16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation).
+//
+// Layout: activations are NHWC bf16 ([batch][y][x][channel]); weights
+// [cout][tap][cin] with tap = 3 ky + kx for the offset (ky - 1, kx - 1);
+// FP32 master weights, bias, gradients and Adam moments.
+//
+// Every convolution is a GEMM over pixels: out[p][n] = sum_k A[p][k] B[k][n].
+//   forward:        A[p][(tap, ci)] = X[p + off(tap)][ci],  B = W[n][tap][ci]
+//   input gradient: A[p][(tap, co)] = dZ[p - off(tap)][co], B = W[co][tap][n]
+//   weight grad.:   dW[co][(tap, ci)] = sum_p dZ[p][co] X[p + off(tap)][ci]
+// A wave computes 16 pixels x 16 channels per MFMA; the B operand (at most
+// 288 x 32 bf16) sits in LDS, transposed so each lane's 4 k-values are one
+// 8-byte read.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ns_closure.h"
+#include "ns_device.h"
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16;
+
+constexpr int kHidden = 32;  // channels of the three hidden layers
+constexpr int kLayers = 4;
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ bf16 f2bf(float f) {  // round to nearest even
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+
+// GELU, tanh form: g(z) = z/2 (1 + tanh(c (z + a z^3))), c = sqrt(2/pi), a = 0.044715
+__device__ __forceinline__ float gelu(float z) {
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    return 0.5f * z * (1.f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_d(float z) {
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    const float t = tanhf(u);
+    return 0.5f * (1.f + t) +
+           0.5f * z * (1.f - t * t) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+}
+
+// pixel p (row-major in an n x n grid, n a power of two) shifted by tap's
+// offset times s, periodic
+__device__ __forceinline__ int shift(int p, int tap, int s, int lg, int n) {
+    const int y = p >> lg, x = p & (n - 1);
+    const int dy = (tap / 3 - 1) * s, dx = (tap % 3 - 1) * s;
+    return (((y + dy) & (n - 1)) << lg) | ((x + dx) & (n - 1));
+}
+
+enum ConvMode { CONV_FWD_GELU = 0, CONV_FWD_LAST = 1, CONV_BWD_DATA = 2 };
+
+struct ConvArgs {
+    int n, lg, npix;
+    const bf16* in;     // A source: [batch][npix][KC]
+    const bf16* w;      // layer weights, bf16 [cout][9][cin]
+    const float* bias;  // [NN] (forward)
+    bf16* z;            // forward: pre-activations [batch][npix][NN]
+    bf16* act;          // forward: GELU(z)
+    float* y;           // last layer: [batch][npix]
+    const bf16* zprev;  // input gradient: pre-activations of the layer below [batch][npix][NN]
+    bf16* dzprev;       // input gradient: dL/dz of the layer below
+};
+
+// KC = channels of the A source, NN = output channels of this GEMM
+template <int KC, int NN, int MODE>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+    constexpr int KP = (9 * KC + 15) / 16 * 16;  // GEMM depth, padded
+    constexpr int NP = (NN + 15) / 16 * 16;      // GEMM width, padded
+    __shared__ __align__(16) bf16 Bt[NP * KP];   // B transposed: Bt[n][k]
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NP * KP; e += 256) {
+        const int nn = e / KP, k = e % KP;
+        float v = 0.f;
+        if (nn < NN && k < 9 * KC) {
+            const int tap = k / KC, kc = k % KC;
+            v = MODE == CONV_BWD_DATA ? bf2f(a.w[(kc * 9 + tap) * NN + nn])
+                                      : bf2f(a.w[(nn * 9 + tap) * KC + kc]);
+        }
+        Bt[e] = f2bf(v);
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const size_t boff = (size_t)blockIdx.y * a.npix;
+    const int p0 = (blockIdx.x * 4 + wave) * 16;
+    const int p = p0 + m;
+    const int s = MODE == CONV_BWD_DATA ? -1 : 1;
+    f4 acc[NP / 16];
+#pragma unroll
+    for (int nb = 0; nb < NP / 16; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < KP / 16; ++c) {
+        const int k0 = c * 16 + 4 * kq;
+        s4 av;
+        if constexpr (KC % 16 == 0) {
+            const int tap = k0 / KC, kc = k0 % KC;
+            const bf16* src = a.in + (boff + shift(p, tap, s, a.lg, a.n)) * KC + kc;
+            av = *reinterpret_cast<const s4*>(src);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = k0 + i;
+                short v = 0;
+                if (k < 9 * KC) {
+                    const int tap = k / KC, kc = k % KC;
+                    v = (short)a.in[(boff + shift(p, tap, s, a.lg, a.n)) * KC + kc];
+                }
+                av[i] = v;
+            }
+        }
+#pragma unroll
+        for (int nb = 0; nb < NP / 16; ++nb) {
+            const s4 bv = *reinterpret_cast<const s4*>(&Bt[(nb * 16 + m) * KP + k0]);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[nb], 0, 0, 0);
+        }
+    }
+    // D[row = 4 kq + i][col = lane & 15]
+#pragma unroll
+    for (int nb = 0; nb < NP / 16; ++nb) {
+        const int nn = nb * 16 + m;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const size_t q = boff + p0 + 4 * kq + i;
+            const float v = acc[nb][i];
+            if constexpr (MODE == CONV_FWD_GELU) {
+                if (nn < NN) {
+                    const float zz = v + a.bias[nn];
+                    a.z[q * NN + nn] = f2bf(zz);
+                    a.act[q * NN + nn] = f2bf(gelu(zz));
+                }
+            } else if constexpr (MODE == CONV_FWD_LAST) {
+                if (nn == 0) a.y[q] = v + a.bias[0];
+            } else {
+                if (nn < NN) a.dzprev[q * NN + nn] = f2bf(v * gelu_d(bf2f(a.zprev[q * NN + nn])));
+            }
+        }
+    }
+}
+
+// dW[co][(tap, ci)] += sum_q dZ[q][co] X[q + off(tap)][ci] over this
+// workgroup's pixel range (all batch members flattened); dbias likewise.
+struct DwArgs {
+    int n, lg, npix, total;  // total = batch * npix
+    int chunk;               // pixels per workgroup (multiple of 16)
+    const bf16* x;           // layer input [total][CIN]
+    const bf16* dz;          // [total][COUT]
+    float* dw;               // [COUT][9][CIN]
+    float* db;               // [COUT]
+};
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
+    constexpr int MT = (9 * CIN + 15) / 16, NT = (COUT + 15) / 16, T = MT * NT;
+    constexpr int TW = (T + 3) / 4;  // tiles per wave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    f4 acc[TW];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    float dbs[NT];  // bias gradient: wave 0, lanes 0..15, channels 16 j + lane
+#pragma unroll
+    for (int j = 0; j < NT; ++j) dbs[j] = 0.f;
+    const int q_begin = blockIdx.x * a.chunk;
+    const int q_end = min(q_begin + a.chunk, a.total);
+    for (int q0 = q_begin; q0 < q_end; q0 += 16) {
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            const int tile = wave + 4 * t;
+            if (tile >= T) continue;
+            const int mb = tile / NT, nb = tile % NT;
+            const int mrow = mb * 16 + r;          // (tap, ci)
+            const int co = nb * 16 + r;
+            s4 av, bv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int q = q0 + 4 * kq + i;
+                short xa = 0, dzv = 0;
+                if (q < q_end) {
+                    const int b = q / a.npix, p = q - b * a.npix;
+                    if (mrow < 9 * CIN) {
+                        const int tap = mrow / CIN, ci = mrow % CIN;
+                        xa = (short)a.x[((size_t)b * a.npix + shift(p, tap, 1, a.lg, a.n)) * CIN + ci];
+                    }
+                    if (co < COUT) dzv = (short)a.dz[(size_t)q * COUT + co];
+                }
+                av[i] = xa;
+                bv[i] = dzv;
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+        }
+        if (wave == 0 && kq == 0) {
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int co = 16 * j + r;
+                if (co < COUT)
+                    for (int e = 0; e < 16 && q0 + e < q_end; ++e)
+                        dbs[j] += bf2f(a.dz[(size_t)(q0 + e) * COUT + co]);
+            }
+        }
+    }
+    if (wave == 0 && kq == 0) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            if (16 * j + r < COUT) atomicAdd(&a.db[16 * j + r], dbs[j]);
+    }
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+        const int tile = wave + 4 * t;
+        if (tile >= T) continue;
+        const int mb = tile / NT, nb = tile % NT;
+        const int co = nb * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int mrow = mb * 16 + 4 * kq + i;
+            if (mrow < 9 * CIN && co < COUT) atomicAdd(&a.dw[co * 9 * CIN + mrow], acc[t][i]);
+        }
+    }
+}
+
+// mean squared error and its gradient dL/dy = 2 (y - t) / total, in bf16
+__global__ void k_mse(int total, const float* y, const float* t, bf16* dy, double* loss) {
+    __shared__ double red[4];
+    double s = 0.0;
+    const float sc = 2.f / (float)total;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const float d = y[q] - t[q];
+        s += (double)d * d;
+        dy[q] = f2bf(sc * d);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) / total);
+}
+
+__global__ void k_to_bf16(size_t count, const float* src, float scale, bf16* dst) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = f2bf(scale * src[i]);
+}
+
+// Adam on the FP32 master parameters; refreshes the bf16 weight copies
+__global__ void k_adam(int count, float* w, const float* g, float* m, float* v, bf16* wb,
+                       const int* is_weight, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (is_weight[i] >= 0) wb[is_weight[i]] = f2bf(wi);
+    }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- closure
+
+struct NsdClosure {
+    int n = 0, lg = 0, npix = 0, max_batch = 0;
+    int cin[kLayers] = {1, kHidden, kHidden, kHidden};
+    int cout[kLayers] = {kHidden, kHidden, kHidden, 1};
+    size_t woff[kLayers] = {}, boff[kLayers] = {}, bwoff[kLayers] = {};
+    size_t nparam = 0, nweight = 0;
+    float* prm = nullptr;    // FP32 master: per layer W then b
+    float* grad = nullptr;
+    float* adm = nullptr;    // Adam first moments
+    float* adv = nullptr;    // Adam second moments
+    int* widx = nullptr;     // parameter -> bf16 weight index (-1 for biases)
+    bf16* wb = nullptr;      // bf16 weights, per layer
+    bf16* x0 = nullptr;      // input [B][npix][1]
+    bf16* z[3] = {};         // hidden pre-activations [B][npix][32]
+    bf16* act[3] = {};       // hidden activations
+    float* y = nullptr;      // output [B][npix]
+    bf16* dz[3] = {};        // gradients of the hidden pre-activations
+    bf16* dy = nullptr;      // gradient of the output [B][npix]
+    float* xin = nullptr;    // FP32 staging [B][npix]
+    float* tin = nullptr;
+    double* loss = nullptr;
+    unsigned long long adam_t = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int ck(NsdClosure* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return 5;  // SOLVER_MEM_ERROR + 1 would collide; SolverStatus has 0..4
+}
+#define CKC(call) \
+    do { if (ck(c, (call), #call)) return 3; } while (0)
+
+unsigned long long splitmix(unsigned long long& s) {
+    unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int launch_forward(NsdClosure* c, int batch) {
+    const dim3 grid(c->npix / 64, batch);
+    ConvArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = c->n;
+    a.lg = c->lg;
+    a.npix = c->npix;
+    for (int l = 0; l < kLayers; ++l) {
+        a.in = l == 0 ? c->x0 : c->act[l - 1];
+        a.w = c->wb + c->bwoff[l];
+        a.bias = c->prm + c->boff[l];
+        if (l < 3) {
+            a.z = c->z[l];
+            a.act = c->act[l];
+        } else {
+            a.y = c->y;
+        }
+        if (l == 0)
+            hipLaunchKernelGGL((k_conv<1, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0, c->stream, a);
+        else if (l < 3)
+            hipLaunchKernelGGL((k_conv<kHidden, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0,
+                               c->stream, a);
+        else
+            hipLaunchKernelGGL((k_conv<kHidden, 1, CONV_FWD_LAST>), grid, dim3(256), 0, c->stream,
+                               a);
+    }
+    return ck(c, hipGetLastError(), "closure forward") ? 3 : 0;
+}
+
+int launch_backward(NsdClosure* c, int batch) {
+    const int total = batch * c->npix;
+    CKC(hipMemsetAsync(c->grad, 0, sizeof(float) * c->nparam, c->stream));
+    // dW, db of layer l from its input and dz_l; then dz_{l-1} = (W_l^T dz_l) * gelu'(z_{l-1})
+    DwArgs d;
+    memset(&d, 0, sizeof(d));
+    d.n = c->n;
+    d.lg = c->lg;
+    d.npix = c->npix;
+    d.total = total;
+    d.chunk = 1024;
+    const int nblk = (total + d.chunk - 1) / d.chunk;
+    ConvArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = c->n;
+    a.lg = c->lg;
+    a.npix = c->npix;
+    const dim3 grid(c->npix / 64, batch);
+    for (int l = kLayers - 1; l >= 0; --l) {
+        const bf16* dzl = l == 3 ? c->dy : c->dz[l];
+        d.x = l == 0 ? c->x0 : c->act[l - 1];
+        d.dz = dzl;
+        d.dw = c->grad + c->woff[l];
+        d.db = c->grad + c->boff[l];
+        if (l == 3)
+            hipLaunchKernelGGL((k_conv_dw<kHidden, 1>), dim3(nblk), dim3(256), 0, c->stream, d);
+        else if (l > 0)
+            hipLaunchKernelGGL((k_conv_dw<kHidden, kHidden>), dim3(nblk), dim3(256), 0, c->stream, d);
+        else
+            hipLaunchKernelGGL((k_conv_dw<1, kHidden>), dim3(nblk), dim3(256), 0, c->stream, d);
+        if (l == 0) break;
+        a.in = dzl;
+        a.w = c->wb + c->bwoff[l];
+        a.zprev = c->z[l - 1];
+        a.dzprev = c->dz[l - 1];
+        if (l == 3)
+            hipLaunchKernelGGL((k_conv<1, kHidden, CONV_BWD_DATA>), grid, dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_conv<kHidden, kHidden, CONV_BWD_DATA>), grid, dim3(256), 0,
+                               c->stream, a);
+    }
+    return ck(c, hipGetLastError(), "closure backward") ? 3 : 0;
+}
+
+int stage_input(NsdClosure* c, const float* x_host, int batch) {
+    const size_t cnt = (size_t)batch * c->npix;
+    CKC(hipMemcpyAsync(c->xin, x_host, sizeof(float) * cnt, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, c->stream, cnt, c->xin, 1.f, c->x0);
+    return 0;
+}
+
+int refresh_bf16(NsdClosure* c) {
+    std::vector<float> h(c->nparam);
+    CKC(hipMemcpy(h.data(), c->prm, sizeof(float) * c->nparam, hipMemcpyDeviceToHost));
+    std::vector<int> wi(c->nparam);
+    CKC(hipMemcpy(wi.data(), c->widx, sizeof(int) * c->nparam, hipMemcpyDeviceToHost));
+    std::vector<bf16> wb(c->nweight);
+    for (size_t i = 0; i < c->nparam; ++i)
+        if (wi[i] >= 0) {
+            unsigned u;
+            memcpy(&u, &h[i], 4);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            wb[wi[i]] = (bf16)(u >> 16);
+        }
+    CKC(hipMemcpy(c->wb, wb.data(), sizeof(bf16) * c->nweight, hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void nsd_closure_destroy(NsdClosure* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->prm, c->grad, c->adm, c->adv, c->widx, c->wb, c->x0, c->z[0], c->z[1],
+                    c->z[2], c->act[0], c->act[1], c->act[2], c->y, c->dz[0], c->dz[1], c->dz[2],
+                    c->dy, c->xin, c->tin, c->loss};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+NsdClosure* nsd_closure_create(int n, int max_batch, unsigned long long seed, char* err,
+                               size_t errlen) {
+    auto fail = [&](const std::string& m, NsdClosure* c) -> NsdClosure* {
+        if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
+        nsd_closure_destroy(c);
+        return nullptr;
+    };
+    if (n < 16 || n > 4096 || (n & (n - 1)) || max_batch < 1)
+        return fail("closure grid must be a power of two in [16, 4096], batch >= 1", nullptr);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device", nullptr);
+    NsdClosure* c = new NsdClosure();
+    c->n = n;
+    while ((1 << c->lg) < n) ++c->lg;
+    c->npix = n * n;
+    c->max_batch = max_batch;
+    size_t off = 0, wo = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        c->woff[l] = off;
+        off += (size_t)c->cout[l] * 9 * c->cin[l];
+        c->boff[l] = off;
+        off += c->cout[l];
+        c->bwoff[l] = wo;
+        wo += (size_t)c->cout[l] * 9 * c->cin[l];
+    }
+    c->nparam = off;
+    c->nweight = wo;
+    const size_t P = (size_t)max_batch * c->npix;
+    auto al = [&](void** p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && al((void**)&c->prm, 4 * off) && al((void**)&c->grad, 4 * off) &&
+         al((void**)&c->adm, 4 * off) && al((void**)&c->adv, 4 * off) &&
+         al((void**)&c->widx, 4 * off) && al((void**)&c->wb, 2 * wo) &&
+         al((void**)&c->x0, 2 * P) && al((void**)&c->y, 4 * P) && al((void**)&c->dy, 2 * P) &&
+         al((void**)&c->xin, 4 * P) && al((void**)&c->tin, 4 * P) &&
+         al((void**)&c->loss, sizeof(double));
+    for (int i = 0; i < 3 && ok; ++i)
+        ok = al((void**)&c->z[i], 2 * P * kHidden) && al((void**)&c->act[i], 2 * P * kHidden) &&
+             al((void**)&c->dz[i], 2 * P * kHidden);
+    if (!ok) return fail("closure: device allocation failed", c);
+    // He-uniform initial weights from a SplitMix64 stream, zero biases
+    std::vector<float> h(off, 0.f);
+    std::vector<int> wi(off, -1);
+    unsigned long long s = seed ? seed : 0x5eedull;
+    for (int l = 0; l < kLayers; ++l) {
+        const size_t cnt = (size_t)c->cout[l] * 9 * c->cin[l];
+        const double lim = std::sqrt(6.0 / (9.0 * c->cin[l]));
+        for (size_t i = 0; i < cnt; ++i) {
+            const double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+            h[c->woff[l] + i] = (float)((2.0 * u - 1.0) * lim);
+            wi[c->woff[l] + i] = (int)(c->bwoff[l] + i);
+        }
+    }
+    if (hipMemcpy(c->prm, h.data(), 4 * off, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->widx, wi.data(), 4 * off, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->adm, 0, 4 * off) != hipSuccess || hipMemset(c->adv, 0, 4 * off) != hipSuccess ||
+        refresh_bf16(c))
+        return fail("closure: initialisation failed", c);
+    return c;
+}
+
+const char* nsd_closure_error(const NsdClosure* c) { return c ? c->err.c_str() : "null closure"; }
+
+int nsd_closure_param_count(const NsdClosure* c) { return c ? (int)c->nparam : 0; }
+
+int nsd_closure_params(NsdClosure* c, float* params, int to_device) {
+    if (!c || !params) return 4;
+    if (to_device) {
+        CKC(hipMemcpy(c->prm, params, 4 * c->nparam, hipMemcpyHostToDevice));
+        return refresh_bf16(c);
+    }
+    CKC(hipMemcpy(params, c->prm, 4 * c->nparam, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int nsd_closure_forward(NsdClosure* c, const float* x, int batch, float* y) {
+    if (!c || !x || !y || batch < 1 || batch > c->max_batch) return 4;
+    if (stage_input(c, x, batch) || launch_forward(c, batch)) return 3;
+    CKC(hipMemcpyAsync(y, c->y, sizeof(float) * batch * c->npix, hipMemcpyDeviceToHost, c->stream));
+    CKC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int nsd_closure_gradient(NsdClosure* c, const float* x, const float* t, int batch, double* loss,
+                         float* grad) {
+    if (!c || !x || !t || batch < 1 || batch > c->max_batch) return 4;
+    const int total = batch * c->npix;
+    if (stage_input(c, x, batch) || launch_forward(c, batch)) return 3;
+    CKC(hipMemcpyAsync(c->tin, t, sizeof(float) * total, hipMemcpyHostToDevice, c->stream));
+    CKC(hipMemsetAsync(c->loss, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, total, c->y, c->tin, c->dy,
+                       c->loss);
+    if (launch_backward(c, batch)) return 3;
+    if (loss) CKC(hipMemcpyAsync(loss, c->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad)
+        CKC(hipMemcpyAsync(grad, c->grad, 4 * c->nparam, hipMemcpyDeviceToHost, c->stream));
+    CKC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"
