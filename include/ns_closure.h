@@ -37,6 +37,31 @@ int nsd_closure_forward(NsdClosure *c, const float *x, int batch, float *y);
 int nsd_closure_gradient(NsdClosure *c, const float *x, const float *t, int batch, double *loss,
                          float *grad);
 
+/* the network sees in_scale * vorticity and its output times out_scale is
+ * the tendency; nsd_closure_train sets both from its data set (unit mean
+ * squares), nsd_closure_set_scales for weights trained elsewhere */
+int nsd_closure_set_scales(NsdClosure *c, double in_scale, double out_scale);
+int nsd_closure_get_scales(const NsdClosure *c, double *in_scale, double *out_scale);
+int nsd_closure_size(const NsdClosure *c);
+
+/* Training data in device memory: `count` pairs (coarse vorticity, subgrid
+ * tendency), each an n x n FP32 field.  nsd_closure_dataset_slot returns
+ * the device addresses of pair `index` (filled by nsd_subgrid_sample of
+ * ns_device.h); nsd_closure_dataset_fill copies host fields in. */
+int nsd_closure_dataset_reserve(NsdClosure *c, int count);
+int nsd_closure_dataset_slot(NsdClosure *c, int index, float **x, float **t);
+int nsd_closure_dataset_fill(NsdClosure *c, int index, const float *x, const float *t);
+int nsd_closure_dataset_read(NsdClosure *c, int index, float *x, float *t);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on the mean squared error over the first
+ * 90 % of the `count` pairs in shuffled minibatches; the last 10 % give the
+ * validation loss.  Losses are in the scaled units (1 = predicting zero). */
+int nsd_closure_train(NsdClosure *c, int count, int epochs, int batch, double lr,
+                      unsigned long long seed, double *loss_first, double *loss_last,
+                      double *loss_valid);
+/* tendency y of the vorticity x, both device FP32 [n][n], queued on the HIP
+ * stream `stream` (capturable into a graph; not re-entrant per network) */
+int nsd_closure_eval_device(NsdClosure *c, const float *x, float *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -133,6 +133,24 @@ const char *nsd_multi_error(const NsdMulti *m);
 int nsd_multi_advance(NsdMulti *m, int nsteps, int adaptive);
 int nsd_multi_synchronize(NsdMulti *m);
 
+/* ---- learned subgrid closure (ns_closure.h); no reference counterpart.
+ * nsd_attach_closure: every step of this context (one FP32 field, square
+ * grid of the network's size) first evaluates the network on the vorticity
+ * and adds its tendency to the nonlinear term of all four stages; NULL
+ * detaches.  nsd_subgrid_sample: from member `member` of a fine context and
+ * a coarse single-field context of the same domain, the coarse vorticity
+ * (the fine modes the coarse 2/3 mask keeps) and the subgrid tendency
+ * filter(N_fine) - N_coarse(filtered vorticity) as physical FP32 fields
+ * (device pointers x, t); the nonlinear fields of all members are computed
+ * when member == 0, so call members in order.  nsd_subgrid_dataset: advance
+ * the fine context `stride` steps, sample every member into the network's
+ * data set from slot `first` on, `snapshots` times. */
+struct NsdClosure;
+int nsd_attach_closure(NsdContext *c, struct NsdClosure *net);
+int nsd_subgrid_sample(NsdContext *fine, int member, NsdContext *coarse, float *x, float *t);
+int nsd_subgrid_dataset(struct NsdClosure *net, NsdContext *fine, NsdContext *coarse,
+                        int snapshots, int stride, int first);
+
 /* ---- compat operations on host arrays (reference layouts, FP64) */
 int nsd_grids(NsdContext *c, double *x, double *y, double *kx, double *ky, double *k2, int *mask);
 int nsd_hv_weights(NsdContext *c, int enabled, double coefficient, int order, double cutoff_ratio,

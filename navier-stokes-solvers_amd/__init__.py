@@ -24,6 +24,7 @@ SOLVER_SUCCESS, SOLVER_MAX_ITER, SOLVER_DIVERGED, SOLVER_MEM_ERROR, SOLVER_INVAL
 NS_PRECISION_FP64, NS_PRECISION_FP32 = 0, 1
 NS_SCHEME_RK4, NS_SCHEME_IF_RK4 = 0, 1
 NS_FORCING_WHITE_NOISE, NS_FORCING_COLORED_NOISE = 0, 1
+NSD_FP64, NSD_FP32 = 0, 1
 
 # every entry point the headers in include/ declare and the library exports
 EXPORTED = [
@@ -74,7 +75,11 @@ EXPORTED = [
     "ns_checkpoint_write_spectral", "ns_checkpoint_read_spectral", "ns_checkpoint_error",
     # ns_closure.h
     "nsd_closure_create", "nsd_closure_destroy", "nsd_closure_error", "nsd_closure_param_count",
-    "nsd_closure_params", "nsd_closure_forward", "nsd_closure_gradient",
+    "nsd_closure_params", "nsd_closure_forward", "nsd_closure_gradient", "nsd_closure_set_scales",
+    "nsd_closure_get_scales", "nsd_closure_size", "nsd_closure_dataset_reserve",
+    "nsd_closure_dataset_slot", "nsd_closure_dataset_fill", "nsd_closure_dataset_read",
+    "nsd_closure_train",
+    "nsd_closure_eval_device", "nsd_attach_closure", "nsd_subgrid_sample", "nsd_subgrid_dataset",
     # vtk_writer.h
     "ns_vtk_write_spectral_legacy", "ns_vtk_write_spectral_xml", "ns_vtk_update_pvd",
     # ns_ensemble.h
@@ -332,6 +337,20 @@ def load():
                                 C.c_int),
         "nsd_closure_gradient": ([V, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                   D, C.POINTER(C.c_float)], C.c_int),
+        "nsd_closure_set_scales": ([V, C.c_double, C.c_double], C.c_int),
+        "nsd_closure_get_scales": ([V, D, D], C.c_int),
+        "nsd_closure_size": ([V], C.c_int),
+        "nsd_closure_dataset_reserve": ([V, C.c_int], C.c_int),
+        "nsd_closure_dataset_slot": ([V, C.c_int, C.POINTER(V), C.POINTER(V)], C.c_int),
+        "nsd_closure_dataset_fill": ([V, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+                                     C.c_int),
+        "nsd_closure_dataset_read": ([V, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+                                     C.c_int),
+        "nsd_closure_train": ([V, C.c_int, C.c_int, C.c_int, C.c_double, ull, D, D, D], C.c_int),
+        "nsd_closure_eval_device": ([V, V, V, V], C.c_int),
+        "nsd_attach_closure": ([V, V], C.c_int),
+        "nsd_subgrid_sample": ([V, C.c_int, V, V, V], C.c_int),
+        "nsd_subgrid_dataset": ([V, V, V, C.c_int, C.c_int, C.c_int], C.c_int),
         "ns_ensemble_config_default": ([C.POINTER(NSEnsembleConfig)], None),
         "ns_ensemble_create": ([C.c_int, I, D, ULLP, C.POINTER(NSEnsembleConfig)], V),
         "ns_ensemble_create_error": ([], C.c_char_p),

@@ -11,8 +11,10 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
+#include "ns_closure.h"
 #include "ns_device.h"
 #include "nsd_stage.hpp"
 
@@ -235,6 +237,37 @@ __global__ void k_spectrum(Geo G, const double* u, const double* v, double* ek, 
             atomicAdd(&ck[bin], 1);
         }
     }
+}
+
+// sharp spectral filter: the modes of a fine reference-layout field that
+// the coarse grid keeps (its 2/3 mask), in the coarse reference layout
+__global__ void k_filter_ref(Geo F, Geo Cg, const double* fine, double* coarse) {
+    const size_t n = (size_t)Cg.nkx * Cg.ny;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int jy = (int)(e / Cg.nkx), i = (int)(e % Cg.nkx);
+        double re = 0.0, im = 0.0;
+        if (kept(Cg, i, jy)) {
+            const int b = band_of(Cg, jy);
+            const size_t fe = (size_t)(b >= 0 ? b : b + F.ny) * F.nkx + i;
+            re = fine[2 * fe];
+            im = fine[2 * fe + 1];
+        }
+        coarse[2 * e] = re;
+        coarse[2 * e + 1] = im;
+    }
+}
+
+__global__ void k_sub_d(size_t n, const double* a, const double* b, double* out) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = a[e] - b[e];
+}
+
+__global__ void k_d2f(size_t n, const double* a, float* out) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = (float)a[e];
 }
 
 // zero the reduction slots of every member's scalars (a kernel, not a
@@ -563,6 +596,11 @@ struct NsdContext {
     double* m_inv_re = nullptr;             // per-member 1/Re (batches), device
     unsigned long long* m_seed = nullptr;   // per-member forcing seeds (batches), device
     unsigned long long* m_ic = nullptr;     // per-member initial-field seeds, device
+    NsdClosure* closure = nullptr;          // learned subgrid tendency (FP32, single field)
+    void* cl_m = nullptr;                   // closure: mixed-layout scratch
+    float* cl_x = nullptr;                  // closure: physical vorticity
+    float* cl_y = nullptr;                  // closure: physical tendency
+    void* cl_src = nullptr;                 // closure: compact spectral tendency
     NsdParams P;
     Phys phys;
     hipStream_t stream = nullptr;
@@ -658,6 +696,7 @@ ColArgs<T> col_args(NsdContext* c) {
     a.bc = C_of(c);
     a.bf = (size_t)c->fbx * (2 * c->fby + 1);
     a.m_inv_re = c->m_inv_re;
+    a.src = c->closure ? (const cx<T>*)c->cl_src : nullptr;
     return a;
 }
 
@@ -805,10 +844,47 @@ void enqueue_force(NsdContext* c) {
     hipLaunchKernelGGL(k_force, dim3(1, c->nb), dim3(256), 0, c->stream, f);
 }
 
+// learned subgrid tendency of the state at the start of the step, into the
+// compact source the stage kernels add to the nonlinear term: inverse
+// transform to physical space, network, forward transform (all on the
+// solver's stream and private buffers, so the step stays capturable)
+template <typename T>
+void enqueue_closure(NsdContext* c) {
+    if constexpr (std::is_same<T, float>::value) {
+        ColArgs<T> a = col_args<T>(c);
+        a.mode = COL_INV;
+        a.in_ref = 0;
+        a.in0 = (const cx<T>*)c->state;
+        a.nout = 1;
+        a.ops[0] = OP_ID0;
+        a.mixed[0] = (cx<T>*)c->cl_m;
+        launch_col_t<T>(c, a, c->G.nkx_keep);
+        RowArgs<T> r = row_args<T>(c);
+        r.mode = ROW_C2R;
+        r.nfields = 1;
+        r.mixed[0] = (const cx<T>*)c->cl_m;
+        r.out[0] = c->cl_x;
+        launch_row_t<T>(c, r, c->G.ny);
+        (void)nsd_closure_eval_device(c->closure, c->cl_x, c->cl_y, (void*)c->stream);
+        RowArgs<T> r2 = row_args<T>(c);
+        r2.mode = ROW_R2C;
+        r2.in_t = c->cl_y;
+        r2.nl = (cx<T>*)c->cl_m;
+        launch_row_t<T>(c, r2, c->G.ny / 2);
+        ColArgs<T> f = col_args<T>(c);
+        f.mode = COL_FWD;
+        f.nl = (const cx<T>*)c->cl_m;
+        f.out_c = (cx<T>*)c->cl_src;
+        f.fwd_scale = 1.0;
+        launch_col_t<T>(c, f, c->G.nkx);
+    }
+}
+
 // one RK4 step: stage 0 computes dt (adaptive) before the forcing draw,
 // which needs dt for the colored-noise decay and is consumed on stage 3
 template <typename T>
 void enqueue_step(NsdContext* c, int adaptive) {
+    if (c->closure) enqueue_closure<T>(c);
     for (int s = 0; s < 4; ++s) {
         enqueue_stage<T>(c, s, adaptive);
         if (s == 0) enqueue_force<T>(c);
@@ -1024,6 +1100,42 @@ int op_state_energy(NsdContext* c, double* out) {
 // holds the derivative fields of the state" is restored by a prime pass
 template <typename T>
 int restore_mixed(NsdContext* c) {
+    return op_prime<T>(c);
+}
+
+// ----------------------------------------------------------------- subgrid samples
+
+// nonlinear term (spectral_core.c:46-112) of member m's state into a device
+// FP64 reference-layout buffer; compute_rows = 0 reuses the J fields of a
+// previous call (the row pass covers every member of a batch at once)
+template <typename T>
+int op_nl_member(NsdContext* c, int m, double* out_ref, int compute_rows) {
+    if (compute_rows) {
+        RowArgs<T> r = row_args<T>(c);
+        r.mode = ROW_STAGE;
+        r.track_umax = 0;
+        launch_row_t<T>(c, r, c->G.ny / 2);
+    }
+    ColArgs<T> f = col_args<T>(c);
+    f.mode = COL_FWD;
+    f.nl = (const cx<T>*)c->nl + (size_t)m * M_of(c);
+    f.out_ref = out_ref;
+    f.fwd_scale = -1.0;  // spectral_core.c:108-111
+    launch_col_t<T>(c, f, c->G.nkx);
+    return lastk(c);
+}
+
+template <typename T>
+int op_scatter_member(NsdContext* c, int m, double* out_ref) {
+    hipLaunchKernelGGL(k_scatter_state<T>, dim3(grid_for(H_of(c))), dim3(256), 0, c->stream, c->G,
+                       (const cx<T>*)c->state + (size_t)m * C_of(c), out_ref);
+    return lastk(c);
+}
+
+template <typename T>
+int op_gather_prime(NsdContext* c, const double* in_ref) {
+    hipLaunchKernelGGL(k_gather_state<T>, dim3(grid_for(C_of(c))), dim3(256), 0, c->stream, c->G,
+                       in_ref, (cx<T>*)c->state);
     return op_prime<T>(c);
 }
 
@@ -1357,7 +1469,8 @@ void nsd_destroy(NsdContext* c) {
     void* bufs[] = {c->state, c->acc, c->stagebuf, c->sfd, c->nl, c->twx, c->twy, c->sc,
                     c->mixed[0], c->mixed[1], c->mixed[2], c->mixed[3], c->stH[0], c->stH[1],
                     c->stH[2], c->stH[3], c->stR[0], c->stR[1], c->stR[2], c->stR[3], c->fmodes,
-                    c->fpartner, c->fou, c->fwork, c->fbox, c->m_inv_re, c->m_seed, c->m_ic};
+                    c->fpartner, c->fou, c->fwork, c->fbox, c->m_inv_re, c->m_seed, c->m_ic,
+                    c->cl_m, c->cl_x, c->cl_y, c->cl_src};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1554,6 +1667,100 @@ int nsd_state_energy(NsdContext* c, double out[2]) {
         return ST_PARAM;
     }
     return TYPED(op_state_energy, c, out);
+}
+
+int nsd_attach_closure(NsdContext* c, NsdClosure* net) {
+    if (!c) return ST_PARAM;
+    if (!net) {
+        c->closure = nullptr;
+        c->gen++;
+        return ST_OK;
+    }
+    if (c->prec != NSD_FP32 || c->nb != 1 || c->G.nx != c->G.ny ||
+        nsd_closure_size(net) != c->G.nx) {
+        c->err = "closure needs a single FP32 field on a square grid of the network's size";
+        return ST_PARAM;
+    }
+    const size_t R = R_of(c);
+    if (!c->cl_m) CK(hipMalloc(&c->cl_m, M_of(c) * c->esz));
+    if (!c->cl_x) CK(hipMalloc(&c->cl_x, sizeof(float) * R));
+    if (!c->cl_y) CK(hipMalloc(&c->cl_y, sizeof(float) * R));
+    if (!c->cl_src) CK(hipMalloc(&c->cl_src, C_of(c) * c->esz));
+    c->closure = net;
+    c->gen++;
+    return ST_OK;
+}
+
+int nsd_subgrid_sample(NsdContext* fine, int member, NsdContext* coarse, float* x, float* t) {
+    if (!fine || !coarse || !x || !t || member < 0 || member >= fine->nb) return ST_PARAM;
+    NsdContext* c = coarse;
+    if (c->nb != 1 || c->G.nx > fine->G.nx || c->G.ny > fine->G.ny || c->G.Lx != fine->G.Lx ||
+        c->G.Ly != fine->G.Ly) {
+        c->err = "subgrid sample: coarse context must be one field on a coarser grid of the same domain";
+        return ST_PARAM;
+    }
+    if (ensure_staging(fine)) {
+        c->err = fine->err;
+        return ST_MEM;
+    }
+    if (ensure_staging(c)) return ST_MEM;
+    {
+        NsdContext* f = fine;
+        int rc = f->prec == NSD_FP32 ? op_nl_member<float>(f, member, f->stH[1], member == 0)
+                                     : op_nl_member<double>(f, member, f->stH[1], member == 0);
+        if (!rc) rc = f->prec == NSD_FP32 ? op_scatter_member<float>(f, member, f->stH[0])
+                                          : op_scatter_member<double>(f, member, f->stH[0]);
+        if (rc) {
+            c->err = f->err;
+            return rc;
+        }
+        const size_t Hc = H_of(c);
+        hipLaunchKernelGGL(k_filter_ref, dim3(grid_for(Hc)), dim3(256), 0, f->stream, f->G, c->G,
+                           f->stH[0], c->stH[0]);
+        hipLaunchKernelGGL(k_filter_ref, dim3(grid_for(Hc)), dim3(256), 0, f->stream, f->G, c->G,
+                           f->stH[1], c->stH[2]);
+        if (lastk(f) || sync(f)) {
+            c->err = f->err;
+            return ST_MEM;
+        }
+    }
+    int rc = TYPED(op_gather_prime, c, c->stH[0]);
+    if (!rc) rc = TYPED(op_nl_member, c, 0, c->stH[1], 1);
+    if (rc) return rc;
+    const size_t Hc = H_of(c), Rc = R_of(c);
+    hipLaunchKernelGGL(k_sub_d, dim3(grid_for(2 * Hc)), dim3(256), 0, c->stream, 2 * Hc,
+                       c->stH[2], c->stH[1], c->stH[3]);
+    const int ops[1] = {OP_ID0};
+    double* o0[1] = {c->stR[0]};
+    double* o1[1] = {c->stR[1]};
+    rc = TYPED(op_inverse, c, c->stH[0], nullptr, 1, ops, o0);
+    if (!rc) rc = TYPED(op_inverse, c, c->stH[3], nullptr, 1, ops, o1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(Rc)), dim3(256), 0, c->stream, Rc, c->stR[0], x);
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(Rc)), dim3(256), 0, c->stream, Rc, c->stR[1], t);
+    if (lastk(c)) return ST_MEM;
+    return sync(c);
+}
+
+int nsd_subgrid_dataset(NsdClosure* net, NsdContext* fine, NsdContext* coarse, int snapshots,
+                        int stride, int first) {
+    if (!net || !fine || !coarse || snapshots < 1 || stride < 0 || first < 0) return ST_PARAM;
+    for (int s = 0; s < snapshots; ++s) {
+        if (stride > 0) {
+            int rc = nsd_advance_graph(fine, stride, 1);
+            if (rc) return rc;
+        }
+        for (int m = 0; m < fine->nb; ++m) {
+            float *x, *t;
+            if (nsd_closure_dataset_slot(net, first + s * fine->nb + m, &x, &t)) {
+                coarse->err = "subgrid data set: slot out of range";
+                return ST_PARAM;
+            }
+            int rc = nsd_subgrid_sample(fine, m, coarse, x, t);
+            if (rc) return rc;
+        }
+    }
+    return ST_OK;
 }
 
 int nsd_batch_members(const NsdContext* c) { return c ? c->nb : 0; }

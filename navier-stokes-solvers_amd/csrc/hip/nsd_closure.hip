@@ -15,6 +15,7 @@
 // 8-byte read.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -71,6 +72,7 @@ struct ConvArgs {
     bf16* z;            // forward: pre-activations [batch][npix][NN]
     bf16* act;          // forward: GELU(z)
     float* y;           // last layer: [batch][npix]
+    float yscale;       // last layer: output multiplied by this
     const bf16* zprev;  // input gradient: pre-activations of the layer below [batch][npix][NN]
     bf16* dzprev;       // input gradient: dL/dz of the layer below
 };
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
                     a.act[q * NN + nn] = f2bf(gelu(zz));
                 }
             } else if constexpr (MODE == CONV_FWD_LAST) {
-                if (nn == 0) a.y[q] = v + a.bias[0];
+                if (nn == 0) a.y[q] = (v + a.bias[0]) * a.yscale;
             } else {
                 if (nn < NN) a.dzprev[q * NN + nn] = f2bf(v * gelu_d(bf2f(a.zprev[q * NN + nn])));
             }
@@ -247,6 +249,41 @@ __global__ void k_mse(int total, const float* y, const float* t, bf16* dy, doubl
     if (threadIdx.x == 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) / total);
 }
 
+// minibatch from the data set: x (scaled, bf16) and target (scaled, FP32)
+__global__ void k_gather_batch(int npix, const int* idx, const float* ds_x, const float* ds_t,
+                               float xs, float ts, bf16* x0, float* t) {
+    const int b = blockIdx.y;
+    const size_t src = (size_t)idx[b] * npix, dst = (size_t)b * npix;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        x0[dst + p] = f2bf(xs * ds_x[src + p]);
+        t[dst + p] = ts * ds_t[src + p];
+    }
+}
+
+// sums of squares of the data set's inputs and targets (for the scales)
+__global__ void k_sumsq(size_t count, const float* x, const float* t, double* out) {
+    __shared__ double red[2][4];
+    double sx = 0.0, st = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x) {
+        sx += (double)x[i] * x[i];
+        st += (double)t[i] * t[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        st += __shfl_xor(st, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sx;
+        red[1][threadIdx.x >> 6] = st;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(&out[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
+}
+
 __global__ void k_to_bf16(size_t count, const float* src, float scale, bf16* dst) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
          i += (size_t)gridDim.x * blockDim.x)
@@ -295,6 +332,12 @@ struct NsdClosure {
     float* tin = nullptr;
     double* loss = nullptr;
     unsigned long long adam_t = 0;
+    float in_scale = 1.f, out_scale = 1.f;  // net input = in_scale x; tendency = out_scale net
+    float* ds_x = nullptr;   // data set: inputs [count][npix]
+    float* ds_t = nullptr;   // data set: targets [count][npix]
+    int ds_count = 0;
+    int* idx = nullptr;      // minibatch indices
+    double* sums = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
 };
@@ -316,13 +359,15 @@ unsigned long long splitmix(unsigned long long& s) {
     return z ^ (z >> 31);
 }
 
-int launch_forward(NsdClosure* c, int batch) {
+// the four layers on `st`, output (times yscale) into y
+int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStream_t st) {
     const dim3 grid(c->npix / 64, batch);
     ConvArgs a;
     memset(&a, 0, sizeof(a));
     a.n = c->n;
     a.lg = c->lg;
     a.npix = c->npix;
+    a.yscale = yscale;
     for (int l = 0; l < kLayers; ++l) {
         a.in = l == 0 ? c->x0 : c->act[l - 1];
         a.w = c->wb + c->bwoff[l];
@@ -331,18 +376,27 @@ int launch_forward(NsdClosure* c, int batch) {
             a.z = c->z[l];
             a.act = c->act[l];
         } else {
-            a.y = c->y;
+            a.y = y;
         }
         if (l == 0)
-            hipLaunchKernelGGL((k_conv<1, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0, c->stream, a);
+            hipLaunchKernelGGL((k_conv<1, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0, st, a);
         else if (l < 3)
-            hipLaunchKernelGGL((k_conv<kHidden, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0,
-                               c->stream, a);
+            hipLaunchKernelGGL((k_conv<kHidden, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0, st, a);
         else
-            hipLaunchKernelGGL((k_conv<kHidden, 1, CONV_FWD_LAST>), grid, dim3(256), 0, c->stream,
-                               a);
+            hipLaunchKernelGGL((k_conv<kHidden, 1, CONV_FWD_LAST>), grid, dim3(256), 0, st, a);
     }
     return ck(c, hipGetLastError(), "closure forward") ? 3 : 0;
+}
+
+int launch_forward(NsdClosure* c, int batch) { return launch_forward_to(c, batch, c->y, 1.f, c->stream); }
+
+int adam_step(NsdClosure* c, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    c->adam_t++;
+    const float c1 = 1.f - powf(b1, (float)c->adam_t), c2 = 1.f - powf(b2, (float)c->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, c->stream, (int)c->nparam, c->prm, c->grad,
+                       c->adm, c->adv, c->wb, c->widx, lr, b1, b2, 1e-8f, c1, c2);
+    return ck(c, hipGetLastError(), "adam") ? 3 : 0;
 }
 
 int launch_backward(NsdClosure* c, int batch) {
@@ -420,9 +474,10 @@ extern "C" {
 void nsd_closure_destroy(NsdClosure* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->prm, c->grad, c->adm, c->adv, c->widx, c->wb, c->x0, c->z[0], c->z[1],
-                    c->z[2], c->act[0], c->act[1], c->act[2], c->y, c->dz[0], c->dz[1], c->dz[2],
-                    c->dy, c->xin, c->tin, c->loss};
+    void* bufs[] = {c->prm,    c->grad,   c->adm,    c->adv,  c->widx,  c->wb,   c->x0,
+                    c->z[0],   c->z[1],   c->z[2],   c->act[0], c->act[1], c->act[2], c->y,
+                    c->dz[0],  c->dz[1],  c->dz[2],  c->dy,   c->xin,   c->tin,  c->loss,
+                    c->ds_x,   c->ds_t,   c->idx,    c->sums};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -464,7 +519,8 @@ NsdClosure* nsd_closure_create(int n, int max_batch, unsigned long long seed, ch
          al((void**)&c->widx, 4 * off) && al((void**)&c->wb, 2 * wo) &&
          al((void**)&c->x0, 2 * P) && al((void**)&c->y, 4 * P) && al((void**)&c->dy, 2 * P) &&
          al((void**)&c->xin, 4 * P) && al((void**)&c->tin, 4 * P) &&
-         al((void**)&c->loss, sizeof(double));
+         al((void**)&c->loss, sizeof(double)) && al((void**)&c->idx, sizeof(int) * max_batch) &&
+         al((void**)&c->sums, 2 * sizeof(double));
     for (int i = 0; i < 3 && ok; ++i)
         ok = al((void**)&c->z[i], 2 * P * kHidden) && al((void**)&c->act[i], 2 * P * kHidden) &&
              al((void**)&c->dz[i], 2 * P * kHidden);
@@ -528,5 +584,133 @@ int nsd_closure_gradient(NsdClosure* c, const float* x, const float* t, int batc
     CKC(hipStreamSynchronize(c->stream));
     return 0;
 }
+
+
+int nsd_closure_set_scales(NsdClosure* c, double in_scale, double out_scale) {
+    if (!c || !(in_scale > 0.0) || !(out_scale > 0.0)) return 4;
+    c->in_scale = (float)in_scale;
+    c->out_scale = (float)out_scale;
+    return 0;
+}
+
+int nsd_closure_get_scales(const NsdClosure* c, double* in_scale, double* out_scale) {
+    if (!c) return 4;
+    if (in_scale) *in_scale = c->in_scale;
+    if (out_scale) *out_scale = c->out_scale;
+    return 0;
+}
+
+int nsd_closure_dataset_reserve(NsdClosure* c, int count) {
+    if (!c || count < 1) return 4;
+    if (c->ds_x) (void)hipFree(c->ds_x);
+    if (c->ds_t) (void)hipFree(c->ds_t);
+    c->ds_x = c->ds_t = nullptr;
+    c->ds_count = 0;
+    const size_t b = sizeof(float) * (size_t)count * c->npix;
+    CKC(hipMalloc(&c->ds_x, b));
+    CKC(hipMalloc(&c->ds_t, b));
+    c->ds_count = count;
+    return 0;
+}
+
+int nsd_closure_dataset_slot(NsdClosure* c, int index, float** x, float** t) {
+    if (!c || index < 0 || index >= c->ds_count || !x || !t) return 4;
+    *x = c->ds_x + (size_t)index * c->npix;
+    *t = c->ds_t + (size_t)index * c->npix;
+    return 0;
+}
+
+int nsd_closure_dataset_fill(NsdClosure* c, int index, const float* x, const float* t) {
+    float *dx, *dt;
+    if (nsd_closure_dataset_slot(c, index, &dx, &dt)) return 4;
+    CKC(hipMemcpy(dx, x, sizeof(float) * c->npix, hipMemcpyHostToDevice));
+    CKC(hipMemcpy(dt, t, sizeof(float) * c->npix, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int nsd_closure_dataset_read(NsdClosure* c, int index, float* x, float* t) {
+    float *dx, *dt;
+    if (!x || !t || nsd_closure_dataset_slot(c, index, &dx, &dt)) return 4;
+    CKC(hipMemcpy(x, dx, sizeof(float) * c->npix, hipMemcpyDeviceToHost));
+    CKC(hipMemcpy(t, dt, sizeof(float) * c->npix, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int nsd_closure_train(NsdClosure* c, int count, int epochs, int batch, double lr,
+                      unsigned long long seed, double* loss_first, double* loss_last,
+                      double* loss_valid) {
+    if (!c || count < 2 || count > c->ds_count || epochs < 1 || batch < 1 ||
+        batch > c->max_batch || !(lr > 0.0))
+        return 4;
+    const size_t cnt = (size_t)count * c->npix;
+    // scales: unit mean square for the network's input and output
+    CKC(hipMemsetAsync(c->sums, 0, 2 * sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, c->stream, cnt, c->ds_x, c->ds_t, c->sums);
+    double sums[2];
+    CKC(hipMemcpyAsync(sums, c->sums, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    CKC(hipStreamSynchronize(c->stream));
+    if (!(sums[0] > 0.0) || !(sums[1] > 0.0)) {
+        c->err = "closure data set is empty or zero";
+        return 4;
+    }
+    c->in_scale = (float)std::sqrt(cnt / sums[0]);
+    c->out_scale = (float)std::sqrt(sums[1] / cnt);
+    const float ts = 1.f / c->out_scale;
+    const int nvalid = count >= 10 ? count / 10 : 1, ntrain = count - nvalid;
+    std::vector<int> order(ntrain);
+    for (int i = 0; i < ntrain; ++i) order[i] = i;
+    unsigned long long s = seed ? seed : 0xc105eull;
+    const dim3 ggrid((c->npix + 255) / 256 < 64 ? (c->npix + 255) / 256 : 64, batch);
+    auto run_batch = [&](const int* ids, int nb, bool learn, double* loss_acc) -> int {
+        CKC(hipMemcpyAsync(c->idx, ids, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_gather_batch, dim3(ggrid.x, nb), dim3(256), 0, c->stream, c->npix,
+                           c->idx, c->ds_x, c->ds_t, c->in_scale, ts, c->x0, c->tin);
+        if (launch_forward(c, nb)) return 3;
+        CKC(hipMemsetAsync(c->loss, 0, sizeof(double), c->stream));
+        hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, nb * c->npix, c->y, c->tin,
+                           c->dy, c->loss);
+        if (learn && (launch_backward(c, nb) || adam_step(c, (float)lr))) return 3;
+        double l = 0.0;
+        CKC(hipMemcpyAsync(&l, c->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CKC(hipStreamSynchronize(c->stream));
+        *loss_acc += l * nb;
+        return 0;
+    };
+    for (int e = 0; e < epochs; ++e) {
+        for (int i = ntrain - 1; i > 0; --i) {  // Fisher-Yates with SplitMix64
+            const int j = (int)(splitmix(s) % (unsigned long long)(i + 1));
+            std::swap(order[i], order[j]);
+        }
+        double acc = 0.0;
+        for (int b0 = 0; b0 + batch <= ntrain; b0 += batch)
+            if (run_batch(&order[b0], batch, true, &acc)) return 3;
+        const double mean = acc / (double)(ntrain / batch * batch);
+        if (e == 0 && loss_first) *loss_first = mean;
+        if (loss_last) *loss_last = mean;
+    }
+    double vacc = 0.0;
+    std::vector<int> vids(batch);
+    int done = 0;
+    for (int b0 = ntrain; b0 < count; b0 += batch) {
+        const int nb = std::min(batch, count - b0);
+        for (int i = 0; i < nb; ++i) vids[i] = b0 + i;
+        if (run_batch(vids.data(), nb, false, &vacc)) return 3;
+        done += nb;
+    }
+    if (loss_valid) *loss_valid = vacc / done;
+    return 0;
+}
+
+// tendency (FP32 [n][n], device) of the vorticity x (FP32 [n][n], device),
+// queued on `stream` (capturable; uses the network's own work buffers)
+int nsd_closure_eval_device(NsdClosure* c, const float* x, float* y, void* stream) {
+    if (!c || !x || !y) return 4;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (size_t)c->npix, x, c->in_scale,
+                       c->x0);
+    return launch_forward_to(c, 1, y, c->out_scale, st);
+}
+
+int nsd_closure_size(const NsdClosure* c) { return c ? c->n : 0; }
 
 }  // extern "C"

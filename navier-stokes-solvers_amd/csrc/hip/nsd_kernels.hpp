@@ -277,6 +277,8 @@ struct ColArgs {
     // strides, on scalars sc[blockIdx.y] and, if set, its own 1/Re
     size_t bm, bc, bf;
     const double* m_inv_re;
+    cx<T>* out_c;      // COL_FWD: compact output of the kept modes (instead of out_ref)
+    const cx<T>* src;  // COL_STAGE: compact source added to the nonlinear term
 };
 
 template <typename T>
@@ -321,6 +323,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     cx<T>* const sfd = a.sfd + mb * a.bc;
     const cx<T>* const fbox = a.fbox + mb * a.bf;
     DevScalars* const sc = a.sc + mb;
+    const cx<T>* const src = a.src ? a.src + mb * a.bc : nullptr;
     const size_t mixed_off = mb * a.bm;
     Phys ph = a.phys;
     if (a.m_inv_re) {
@@ -359,8 +362,13 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             const int jy = j + s * TPL;
-            cx<T> r = (has_data && kept(G, kxi, jy)) ? norm * v[s] : zero;
-            if (active) st_ref(a.out_ref, (size_t)jy * G.nkx + kxi, r);
+            const bool k = has_data && kept(G, kxi, jy);
+            cx<T> r = k ? norm * v[s] : zero;
+            if (a.out_c) {  // compact layout of the kept modes
+                if (k) a.out_c[(size_t)kxi * G.nky_keep + compact_y(G, jy)] = r;
+            } else if (active) {
+                st_ref(a.out_ref, (size_t)jy * G.nkx + kxi, r);
+            }
         }
         return;
     }
@@ -406,7 +414,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             continue;
         }
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
-        const cx<T> N = (-norm) * v[s];
+        cx<T> N = (-norm) * v[s];
+        if (src) N = N + src[ci];  // learned subgrid tendency, frozen over the step
         const cx<T> w0 = state[ci];
         const T k2 = kx * kx + ky * ky;
         const T kmag = sqrt(k2);
@@ -573,6 +582,7 @@ struct RowArgs {
     double* out_ref[4];    // ROW_C2R outputs (FP64 [ny][nx]) if non-null
     T* out[4];             // ROW_C2R outputs (T [ny][nx]) otherwise
     const double* in_ref;  // ROW_R2C input (FP64 [ny][nx])
+    const T* in_t;         // ROW_R2C input at the working precision (instead of in_ref)
     DevScalars* sc;
     size_t bm;             // batches: mixed-field stride of member blockIdx.y
 };
@@ -755,9 +765,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             const int x = j + s * TPL;
-            v[s] = active ? mk(T(a.in_ref[(size_t)y0 * G.nx + x]),
-                               T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]))
-                          : mk(T(0), T(0));
+            if (!active) v[s] = mk(T(0), T(0));
+            else if (a.in_t) v[s] = mk(a.in_t[(size_t)y0 * G.nx + x], a.in_t[(size_t)(y0 + 1) * G.nx + x]);
+            else v[s] = mk(T(a.in_ref[(size_t)y0 * G.nx + x]), T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]));
         }
         fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
         return;

@@ -75,3 +75,127 @@ def test_gradient_matches_oracle(ns, n, batch):
         assert ew <= 2e-2 and eb <= 2e-2
         off += nw + co
     L.nsd_closure_destroy(h)
+
+
+# ------------------------------------------------------------ subgrid data
+
+def ctx_of(ns, n, fp32, Re=1e3):
+    s = ns.Spectral(n, n, 2 * np.pi, 2 * np.pi, Re)
+    s.init(taylor_green=False)
+    s.set_precision(fp32)
+    return s
+
+
+def test_subgrid_sample_matches_oracle(ns, oc):
+    """x = filtered vorticity, t = filter(N_fine) - N_coarse(x), in physical
+    space on the coarse grid, against the oracle's nonlinear term and
+    transforms (FP64 contexts; outputs are FP32)."""
+    nf, nc = 64, 32
+    f = ctx_of(ns, nf, False)
+    f.random_phase(21, k_peak=6.0)
+    f.sync_device()
+    c = ctx_of(ns, nc, False)
+    c.sync_device()
+    L = ns.load()
+    x = np.zeros((nc, nc), np.float32)
+    t = np.zeros((nc, nc), np.float32)
+    # device outputs: a network's data-set slot
+    err = C.create_string_buffer(256)
+    net = L.nsd_closure_create(nc, 1, 1, err, 256)
+    assert net and L.nsd_closure_dataset_reserve(net, 1) == 0
+    px, pt = C.c_void_p(), C.c_void_p()
+    assert L.nsd_closure_dataset_slot(net, 0, C.byref(px), C.byref(pt)) == 0
+    assert L.nsd_subgrid_sample(f.ctx, 0, c.ctx, px, pt) == 0, c.L.nsd_error(c.ctx)
+    assert L.nsd_closure_dataset_read(net, 0, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                      t.ctypes.data_as(C.POINTER(C.c_float))) == 0
+    # oracle
+    of = oc.Oracle(nf, nf, 2 * np.pi, 2 * np.pi, 1e3)
+    of.init_grids()
+    of.init_random_phase(21, 6.0, 1.0)
+    wf = of.spec("omega_hat").copy()
+    of.velocity_from_vorticity()
+    of.nonlinear()
+    Nf = of.spec("nonlinear_hat").copy()
+    o = oc.Oracle(nc, nc, 2 * np.pi, 2 * np.pi, 1e3)
+    o.init_grids()
+    mask = o.mask.astype(bool)
+    kyc = np.where(np.arange(nc) <= nc // 2, np.arange(nc), np.arange(nc) - nc)
+
+    def filt(a):
+        out = np.zeros((nc, nc // 2 + 1), complex)
+        rows = np.where(kyc >= 0, kyc, kyc + nf)
+        out[:, :] = a[rows, : nc // 2 + 1]
+        return np.where(mask, out, 0)
+
+    wbar = filt(wf)
+    o.spec("omega_hat")[:] = wbar
+    o.velocity_from_vorticity()
+    o.nonlinear()
+    S = filt(Nf) - o.spec("nonlinear_hat")
+    x_ref = o.c2r(wbar)
+    t_ref = o.c2r(S)
+    ex, et = rel_l2(x, x_ref), rel_l2(t, t_ref)
+    print(f"sample vs oracle: vorticity {ex:.1e}, subgrid tendency {et:.1e}", flush=True)
+    assert ex <= 1e-6 and et <= 1e-6
+    L.nsd_closure_destroy(net)
+
+
+def test_closure_source_enters_the_step(ns):
+    """A network whose only non-zero parameter is the last bias b outputs the
+    constant tendency b * out_scale; without drag the mean vorticity (the
+    k = 0 mode, which the nonlinear term never changes) then grows as
+    b * out_scale * t."""
+    n = 64
+    s = ctx_of(ns, n, True, Re=500.0)
+    s.set_scheme(True)
+    s.random_phase(5, k_peak=6.0)
+    s.d.dt = 1e-3
+    s.sync_device()
+    L = ns.load()
+    err = C.create_string_buffer(256)
+    net = L.nsd_closure_create(n, 1, 1, err, 256)
+    P = L.nsd_closure_param_count(net)
+    p = np.zeros(P, np.float32)
+    p[-1] = 0.25
+    assert L.nsd_closure_params(net, fp(p), 1) == 0
+    assert L.nsd_closure_set_scales(net, 1.0, 2.0) == 0
+    assert L.nsd_attach_closure(s.ctx, net) == 0
+    assert s.advance_n(40, False) == 0
+    assert s.sync_host() == 0
+    w00 = s.spec("omega_hat")[0, 0]
+    print(f"mean vorticity {w00.real:.7f} at t = {s.d.t:.4f} (expected {0.5 * s.d.t:.7f})", flush=True)
+    assert abs(w00.real - 0.5 * s.d.t) <= 1e-5 and abs(w00.imag) <= 1e-6
+    assert L.nsd_attach_closure(s.ctx, None) == 0
+    s.close()
+    L.nsd_closure_destroy(net)
+
+
+def test_dataset_and_training_reduce_the_loss(ns):
+    nf, nc, members = 256, 64, 4
+    L = ns.load()
+    err = C.create_string_buffer(256)
+    fine = L.nsd_create_batch(nf, nf, 2 * np.pi, 2 * np.pi, ns.NSD_FP32, members, err, 256)
+    assert fine, err.value
+    p = ns.NsdParams()
+    p.Re, p.drag, p.cfl_factor, p.scheme = 1e4, 0.1, 0.5, 1
+    kmax = np.sqrt(2.0) * nf / 2
+    p.hv_on, p.hv_order, p.hv_coeff, p.hv_kc, p.hv_invbw = 1, 8, 1e-16 * kmax ** 8, 0.0, 1 / kmax
+    p.force_on, p.force_kmin, p.force_kmax, p.force_energy_target, p.force_rate = 1, 4.0, 6.0, 1.0, 1e-3
+    assert L.nsd_set_params(fine, C.byref(p)) == 0
+    seeds = (C.c_ulonglong * members)(*range(1, members + 1))
+    assert L.nsd_set_member_params(fine, None, seeds) == 0
+    assert L.nsd_batch_random_phase(fine, seeds, 10.0, 1.0) == 0
+    coarse = L.nsd_create(nc, nc, 2 * np.pi, 2 * np.pi, ns.NSD_FP64, err, 256)
+    assert coarse, err.value
+    net = L.nsd_closure_create(nc, 8, 7, err, 256)
+    snaps = 16
+    assert L.nsd_closure_dataset_reserve(net, snaps * members) == 0
+    assert L.nsd_subgrid_dataset(net, fine, coarse, snaps, 10, 0) == 0, L.nsd_error(coarse)
+    l0, l1, lv = C.c_double(), C.c_double(), C.c_double()
+    assert L.nsd_closure_train(net, snaps * members, 30, 8, 2e-3, 3, C.byref(l0), C.byref(l1),
+                               C.byref(lv)) == 0, L.nsd_closure_error(net)
+    print(f"training loss {l0.value:.3f} -> {l1.value:.3f}, validation {lv.value:.3f}", flush=True)
+    assert l1.value < 0.7 * l0.value and lv.value < 1.0
+    for h in (fine, coarse):
+        L.nsd_destroy(h)
+    L.nsd_closure_destroy(net)
