@@ -52,15 +52,19 @@ int nsd_closure_dataset_reserve(NsdClosure *c, int count);
 int nsd_closure_dataset_slot(NsdClosure *c, int index, float **x, float **t);
 int nsd_closure_dataset_fill(NsdClosure *c, int index, const float *x, const float *t);
 int nsd_closure_dataset_read(NsdClosure *c, int index, float *x, float *t);
-/* Adam (beta 0.9 / 0.999, eps 1e-8) on the mean squared error over the first
- * 90 % of the `count` pairs in shuffled minibatches; the last 10 % give the
- * validation loss.  Losses are in the scaled units (1 = predicting zero). */
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on the mean squared error over 90 % of
+ * the `count` pairs in shuffled minibatches; every tenth pair is held out
+ * for the validation loss.  Losses are in the scaled units (1 = predicting
+ * zero). */
 int nsd_closure_train(NsdClosure *c, int count, int epochs, int batch, double lr,
                       unsigned long long seed, double *loss_first, double *loss_last,
                       double *loss_valid);
 /* tendency y of the vorticity x, both device FP32 [n][n], queued on the HIP
  * stream `stream` (capturable into a graph; not re-entrant per network) */
 int nsd_closure_eval_device(NsdClosure *c, const float *x, float *y, void *stream);
+/* the same for `batch` fields [batch][n][n] (batch <= the network's max_batch) */
+int nsd_closure_eval_batch(NsdClosure *c, const float *x, float *y, int batch, void *stream);
+int nsd_closure_max_batch(const NsdClosure *c);
 
 #ifdef __cplusplus
 }

@@ -134,8 +134,9 @@ int nsd_multi_advance(NsdMulti *m, int nsteps, int adaptive);
 int nsd_multi_synchronize(NsdMulti *m);
 
 /* ---- learned subgrid closure (ns_closure.h); no reference counterpart.
- * nsd_attach_closure: every step of this context (one FP32 field, square
- * grid of the network's size) first evaluates the network on the vorticity
+ * nsd_attach_closure: every step of this context (FP32 fields on a square
+ * grid of the network's size; a batch context needs a network batch of at
+ * least its members) first evaluates the network on each member's vorticity
  * and adds its tendency to the nonlinear term of all four stages; NULL
  * detaches.  nsd_subgrid_sample: from member `member` of a fine context and
  * a coarse single-field context of the same domain, the coarse vorticity

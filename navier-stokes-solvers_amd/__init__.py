@@ -79,7 +79,8 @@ EXPORTED = [
     "nsd_closure_get_scales", "nsd_closure_size", "nsd_closure_dataset_reserve",
     "nsd_closure_dataset_slot", "nsd_closure_dataset_fill", "nsd_closure_dataset_read",
     "nsd_closure_train",
-    "nsd_closure_eval_device", "nsd_attach_closure", "nsd_subgrid_sample", "nsd_subgrid_dataset",
+    "nsd_closure_eval_device", "nsd_closure_eval_batch", "nsd_closure_max_batch",
+    "nsd_attach_closure", "nsd_subgrid_sample", "nsd_subgrid_dataset",
     # vtk_writer.h
     "ns_vtk_write_spectral_legacy", "ns_vtk_write_spectral_xml", "ns_vtk_update_pvd",
     # ns_ensemble.h
@@ -348,6 +349,8 @@ def load():
                                      C.c_int),
         "nsd_closure_train": ([V, C.c_int, C.c_int, C.c_int, C.c_double, ull, D, D, D], C.c_int),
         "nsd_closure_eval_device": ([V, V, V, V], C.c_int),
+        "nsd_closure_eval_batch": ([V, V, V, C.c_int, V], C.c_int),
+        "nsd_closure_max_batch": ([V], C.c_int),
         "nsd_attach_closure": ([V, V], C.c_int),
         "nsd_subgrid_sample": ([V, C.c_int, V, V, V], C.c_int),
         "nsd_subgrid_dataset": ([V, V, V, C.c_int, C.c_int, C.c_int], C.c_int),

@@ -715,12 +715,11 @@ RowArgs<T> row_args(NsdContext* c) {
 
 template <typename T, int NT>
 void launch_col_nt(NsdContext* c, const ColArgs<T>& a, int blocks, size_t lds) {
+    const dim3 grid(blocks, a.nbat > 1 ? a.nbat : 1);
     if (a.mode == COL_FWD)
-        hipLaunchKernelGGL((k_col<T, NT, COL_FWD, 0, 0>), dim3(blocks), dim3(NT), lds, c->stream,
-                           a);
+        hipLaunchKernelGGL((k_col<T, NT, COL_FWD, 0, 0>), grid, dim3(NT), lds, c->stream, a);
     else
-        hipLaunchKernelGGL((k_col<T, NT, COL_INV, 0, 0>), dim3(blocks), dim3(NT), lds, c->stream,
-                           a);
+        hipLaunchKernelGGL((k_col<T, NT, COL_INV, 0, 0>), grid, dim3(NT), lds, c->stream, a);
 }
 
 template <typename T>
@@ -741,10 +740,11 @@ void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
 
 template <typename T, int NT>
 void launch_row_nt(NsdContext* c, const RowArgs<T>& a, int blocks, size_t lds) {
+    const dim3 grid(blocks, a.nbat > 1 ? a.nbat : 1);
     if (a.mode == ROW_C2R)
-        hipLaunchKernelGGL((k_row<T, NT, ROW_C2R, 0>), dim3(blocks), dim3(NT), lds, c->stream, a);
+        hipLaunchKernelGGL((k_row<T, NT, ROW_C2R, 0>), grid, dim3(NT), lds, c->stream, a);
     else
-        hipLaunchKernelGGL((k_row<T, NT, ROW_R2C, 0>), dim3(blocks), dim3(NT), lds, c->stream, a);
+        hipLaunchKernelGGL((k_row<T, NT, ROW_R2C, 0>), grid, dim3(NT), lds, c->stream, a);
 }
 
 template <typename T>
@@ -851,8 +851,10 @@ void enqueue_force(NsdContext* c) {
 template <typename T>
 void enqueue_closure(NsdContext* c) {
     if constexpr (std::is_same<T, float>::value) {
+        const int nb = c->nb;
         ColArgs<T> a = col_args<T>(c);
         a.mode = COL_INV;
+        a.nbat = nb;
         a.in_ref = 0;
         a.in0 = (const cx<T>*)c->state;
         a.nout = 1;
@@ -861,18 +863,23 @@ void enqueue_closure(NsdContext* c) {
         launch_col_t<T>(c, a, c->G.nkx_keep);
         RowArgs<T> r = row_args<T>(c);
         r.mode = ROW_C2R;
+        r.nbat = nb;
+        r.br = R_of(c);
         r.nfields = 1;
         r.mixed[0] = (const cx<T>*)c->cl_m;
         r.out[0] = c->cl_x;
         launch_row_t<T>(c, r, c->G.ny);
-        (void)nsd_closure_eval_device(c->closure, c->cl_x, c->cl_y, (void*)c->stream);
+        (void)nsd_closure_eval_batch(c->closure, c->cl_x, c->cl_y, nb, (void*)c->stream);
         RowArgs<T> r2 = row_args<T>(c);
         r2.mode = ROW_R2C;
+        r2.nbat = nb;
+        r2.br = R_of(c);
         r2.in_t = c->cl_y;
         r2.nl = (cx<T>*)c->cl_m;
         launch_row_t<T>(c, r2, c->G.ny / 2);
         ColArgs<T> f = col_args<T>(c);
         f.mode = COL_FWD;
+        f.nbat = nb;
         f.nl = (const cx<T>*)c->cl_m;
         f.out_c = (cx<T>*)c->cl_src;
         f.fwd_scale = 1.0;
@@ -1676,16 +1683,17 @@ int nsd_attach_closure(NsdContext* c, NsdClosure* net) {
         c->gen++;
         return ST_OK;
     }
-    if (c->prec != NSD_FP32 || c->nb != 1 || c->G.nx != c->G.ny ||
-        nsd_closure_size(net) != c->G.nx) {
-        c->err = "closure needs a single FP32 field on a square grid of the network's size";
+    if (c->prec != NSD_FP32 || c->G.nx != c->G.ny || nsd_closure_size(net) != c->G.nx ||
+        nsd_closure_max_batch(net) < c->nb) {
+        c->err = "closure needs FP32 fields on a square grid of the network's size, and a "
+                 "network batch of at least the context's members";
         return ST_PARAM;
     }
-    const size_t R = R_of(c);
-    if (!c->cl_m) CK(hipMalloc(&c->cl_m, M_of(c) * c->esz));
+    const size_t R = R_of(c) * c->nb;
+    if (!c->cl_m) CK(hipMalloc(&c->cl_m, M_of(c) * c->esz * c->nb));
     if (!c->cl_x) CK(hipMalloc(&c->cl_x, sizeof(float) * R));
     if (!c->cl_y) CK(hipMalloc(&c->cl_y, sizeof(float) * R));
-    if (!c->cl_src) CK(hipMalloc(&c->cl_src, C_of(c) * c->esz));
+    if (!c->cl_src) CK(hipMalloc(&c->cl_src, C_of(c) * c->esz * c->nb));
     c->closure = net;
     c->gen++;
     return ST_OK;

@@ -656,9 +656,14 @@ int nsd_closure_train(NsdClosure* c, int count, int epochs, int batch, double lr
     c->in_scale = (float)std::sqrt(cnt / sums[0]);
     c->out_scale = (float)std::sqrt(sums[1] / cnt);
     const float ts = 1.f / c->out_scale;
-    const int nvalid = count >= 10 ? count / 10 : 1, ntrain = count - nvalid;
-    std::vector<int> order(ntrain);
-    for (int i = 0; i < ntrain; ++i) order[i] = i;
+    // every tenth pair is held out for validation (spread over members and times)
+    std::vector<int> order, valid;
+    for (int i = 0; i < count; ++i) (i % 10 == 9 ? valid : order).push_back(i);
+    if (valid.empty()) {
+        valid.push_back(order.back());
+        order.pop_back();
+    }
+    const int ntrain = (int)order.size();
     unsigned long long s = seed ? seed : 0xc105eull;
     const dim3 ggrid((c->npix + 255) / 256 < 64 ? (c->npix + 255) / 256 : 64, batch);
     auto run_batch = [&](const int* ids, int nb, bool learn, double* loss_acc) -> int {
@@ -689,27 +694,31 @@ int nsd_closure_train(NsdClosure* c, int count, int epochs, int batch, double lr
         if (loss_last) *loss_last = mean;
     }
     double vacc = 0.0;
-    std::vector<int> vids(batch);
     int done = 0;
-    for (int b0 = ntrain; b0 < count; b0 += batch) {
-        const int nb = std::min(batch, count - b0);
-        for (int i = 0; i < nb; ++i) vids[i] = b0 + i;
-        if (run_batch(vids.data(), nb, false, &vacc)) return 3;
+    for (size_t b0 = 0; b0 < valid.size(); b0 += batch) {
+        const int nb = (int)std::min<size_t>(batch, valid.size() - b0);
+        if (run_batch(&valid[b0], nb, false, &vacc)) return 3;
         done += nb;
     }
     if (loss_valid) *loss_valid = vacc / done;
     return 0;
 }
 
-// tendency (FP32 [n][n], device) of the vorticity x (FP32 [n][n], device),
+// tendencies (FP32 [batch][n][n], device) of the vorticities x (same shape),
 // queued on `stream` (capturable; uses the network's own work buffers)
-int nsd_closure_eval_device(NsdClosure* c, const float* x, float* y, void* stream) {
-    if (!c || !x || !y) return 4;
+int nsd_closure_eval_batch(NsdClosure* c, const float* x, float* y, int batch, void* stream) {
+    if (!c || !x || !y || batch < 1 || batch > c->max_batch) return 4;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (size_t)c->npix, x, c->in_scale,
-                       c->x0);
-    return launch_forward_to(c, 1, y, c->out_scale, st);
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (size_t)batch * c->npix, x,
+                       c->in_scale, c->x0);
+    return launch_forward_to(c, batch, y, c->out_scale, st);
 }
+
+int nsd_closure_eval_device(NsdClosure* c, const float* x, float* y, void* stream) {
+    return nsd_closure_eval_batch(c, x, y, 1, stream);
+}
+
+int nsd_closure_max_batch(const NsdClosure* c) { return c ? c->max_batch : 0; }
 
 int nsd_closure_size(const NsdClosure* c) { return c ? c->n : 0; }
 

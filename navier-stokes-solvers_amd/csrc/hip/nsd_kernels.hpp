@@ -279,6 +279,7 @@ struct ColArgs {
     const double* m_inv_re;
     cx<T>* out_c;      // COL_FWD: compact output of the kept modes (instead of out_ref)
     const cx<T>* src;  // COL_STAGE: compact source added to the nonlinear term
+    int nbat;          // members along grid y for the transform modes (0 = 1)
 };
 
 template <typename T>
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const Geo& G = a.geo;
     // member blockIdx.y of a batch: its fields, scalars and 1/Re (the
     // argument block itself stays untouched, in constant kernarg memory)
-    const size_t mb = (MODE == COL_STAGE || MODE == COL_PRIME) ? blockIdx.y : 0;
+    const size_t mb = blockIdx.y;
     const cx<T>* const nl = a.nl + mb * a.bm;
     cx<T>* const state = a.state + mb * a.bc;
     cx<T>* const acc = a.acc + mb * a.bc;
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             const bool k = has_data && kept(G, kxi, jy);
             cx<T> r = k ? norm * v[s] : zero;
             if (a.out_c) {  // compact layout of the kept modes
-                if (k) a.out_c[(size_t)kxi * G.nky_keep + compact_y(G, jy)] = r;
+                if (k) a.out_c[mb * a.bc + (size_t)kxi * G.nky_keep + compact_y(G, jy)] = r;
             } else if (active) {
                 st_ref(a.out_ref, (size_t)jy * G.nkx + kxi, r);
             }
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             continue;
         }
         if constexpr (MODE == COL_INV) {
-            w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : a.in0[ci];
+            w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : a.in0[mb * a.bc + ci];
             continue;
         }
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
@@ -585,6 +586,8 @@ struct RowArgs {
     const T* in_t;         // ROW_R2C input at the working precision (instead of in_ref)
     DevScalars* sc;
     size_t bm;             // batches: mixed-field stride of member blockIdx.y
+    size_t br;             // batches: physical-field stride (ROW_C2R out, ROW_R2C in_t)
+    int nbat;              // members along grid y for the transform modes (0 = 1)
 };
 
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
@@ -733,24 +736,25 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     cx<T> v[P];
     // member blockIdx.y of a batch (ROW_STAGE only)
-    const size_t moff = (MODE == ROW_STAGE) ? blockIdx.y * a.bm : 0;
+    // member blockIdx.y of a batch (mixed fields, physical fields)
+    const size_t moff = blockIdx.y * a.bm, roff = blockIdx.y * a.br;
 
     if constexpr (MODE == ROW_C2R) {
         const int y = item;
         const bool active = y < G.ny;
         for (int f = 0; f < a.nfields; f += 2) {
-            const cx<T>* B = (f + 1 < a.nfields) ? a.mixed[f + 1] : nullptr;
-            build_row_pair<LC>(v, lds, j, g, G, a.mixed[f], B, y, active);
+            const cx<T>* B = (f + 1 < a.nfields) ? a.mixed[f + 1] + moff : nullptr;
+            build_row_pair<LC>(v, lds, j, g, G, a.mixed[f] + moff, B, y, active);
             fft_any<+1, LC>(v, lds, j, g, a.tw);
             if (active) {
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
                     const size_t o = (size_t)y * G.nx + j + s * TPL;
                     if (a.out_ref[f]) a.out_ref[f][o] = double(v[s].x);
-                    else a.out[f][o] = v[s].x;
+                    else a.out[f][roff + o] = v[s].x;
                     if (B) {
                         if (a.out_ref[f + 1]) a.out_ref[f + 1][o] = double(v[s].y);
-                        else a.out[f + 1][o] = v[s].y;
+                        else a.out[f + 1][roff + o] = v[s].y;
                     }
                 }
             }
@@ -766,10 +770,10 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         for (int s = 0; s < P; ++s) {
             const int x = j + s * TPL;
             if (!active) v[s] = mk(T(0), T(0));
-            else if (a.in_t) v[s] = mk(a.in_t[(size_t)y0 * G.nx + x], a.in_t[(size_t)(y0 + 1) * G.nx + x]);
+            else if (a.in_t) v[s] = mk(a.in_t[roff + (size_t)y0 * G.nx + x], a.in_t[roff + (size_t)(y0 + 1) * G.nx + x]);
             else v[s] = mk(T(a.in_ref[(size_t)y0 * G.nx + x]), T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]));
         }
-        fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl, y0, active);
+        fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
         return;
     }
 

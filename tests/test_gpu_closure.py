@@ -199,3 +199,28 @@ def test_dataset_and_training_reduce_the_loss(ns):
     for h in (fine, coarse):
         L.nsd_destroy(h)
     L.nsd_closure_destroy(net)
+
+
+def test_trained_closure_beats_the_plain_coarse_solve(ns):
+    """A posteriori: coarse runs with the trained network against the plain
+    coarse solve, both against the filtered fine ensemble (1024^2 -> 128^2,
+    8 members): the closure must at least halve the error of the ensemble
+    mean spectrum, keep enstrophy within the 10 % acceptance bound, and be
+    accepted (else the workflow falls back to the plain solve)."""
+    import importlib
+    cl = importlib.import_module(ns.__name__ + ".closure")
+    net = cl.Net(128, max_batch=16, seed=1)
+    seeds = list(range(1, 33))
+    net.reserve(len(seeds) * 32)
+    cl.generate(net, 1024, seeds, 32, 10)
+    l0, l1, lv = net.train(8, batch=16, lr=2e-3)
+    r = cl.evaluate(net, 1024, list(range(2001, 2009)), 0.3)
+    print(f"loss {l0:.3f} -> {l1:.3f} (validation {lv:.3f}); plain: spectrum "
+          f"{r['plain']['spectrum_mean_rel_err']:.3f}, enstrophy {r['plain']['enstrophy_rel_err']:.3f}; "
+          f"closure: spectrum {r['closure']['spectrum_mean_rel_err']:.3f}, enstrophy "
+          f"{r['closure']['enstrophy_rel_err']:.3f}, {r['closure']['speedup_vs_fine']:.1f}x the fine runs",
+          flush=True)
+    assert lv < 1.0
+    assert r["closure"]["spectrum_mean_rel_err"] < 0.5 * r["plain"]["spectrum_mean_rel_err"]
+    assert r["closure_accepted"] and r["result"] == "closure"
+    net.close()
