@@ -140,6 +140,38 @@ def time_ensemble(ns, steps, warmup, members=256):
             "all_status_ok": bool(np.all(st == 0))}
 
 
+def closure_workload(ns, seeds=64, snapshots=64, stride=10, epochs=10, members=16, t_end=0.5):
+    """Learned closure: fine 2048^2 forced runs (one per seed) sampled every
+    `stride` steps into (coarse vorticity, subgrid tendency) pairs at 256^2,
+    Adam training of the bf16 MFMA network, then 256^2 ensembles with and
+    without it against the filtered 2048^2 ensemble from the same initial
+    fields (same physics as the 4096^2 workload)."""
+    import importlib
+    cl = importlib.import_module(ns.__name__ + ".closure")
+    net = cl.Net(256, max_batch=max(16, members), seed=1)
+    t0 = time.perf_counter()
+    net.reserve(seeds * snapshots)
+    cl.generate(net, 2048, list(range(1, seeds + 1)), snapshots, stride)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    l0, l1, lv = net.train(epochs, batch=16, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    r = cl.evaluate(net, 2048, list(range(1001, 1001 + members)), t_end)
+    net.close()
+    return {"workload": f"2048^2 -> 256^2, {seeds} seeds x {snapshots} snapshots every "
+                        f"{stride} steps, {epochs} epochs; evaluation: {members}-member "
+                        f"ensembles to t = {r['t_end']:.3f}",
+            "speedup_vs_fine": round(r["closure"]["speedup_vs_fine"], 2),
+            "spectrum_mean_rel_err": round(r["closure"]["spectrum_mean_rel_err"], 4),
+            "enstrophy_rel_err": round(r["closure"]["enstrophy_rel_err"], 4),
+            "plain_coarse": {"spectrum_mean_rel_err": round(r["plain"]["spectrum_mean_rel_err"], 4),
+                             "enstrophy_rel_err": round(r["plain"]["enstrophy_rel_err"], 4),
+                             "speedup_vs_fine": round(r["plain"]["speedup_vs_fine"], 2)},
+            "result": r["result"], "loss": [round(l0, 4), round(l1, 4)],
+            "validation_loss": round(lv, 4), "data_s": round(t_data, 1),
+            "train_s": round(t_train, 1)}
+
+
 def cpu_baseline(n):
     """Oracle (reference algorithm, project FFT, OpenMP) on this host: one step
     of the same forced integrating-factor workload at FP64."""
@@ -171,6 +203,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip training and evaluating the learned closure (about 30 s)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-member mixed-size ensemble")
     ap.add_argument("--no-compat", action="store_true",
@@ -242,6 +276,8 @@ def main():
                        "row_ms": round(r64["row_ms"], 4), "col_ms": round(r64["col_ms"], 4)}
     if not a.no_ensemble:
         out["ensemble"] = time_ensemble(ns, max(a.steps // 2, 5), 3)
+    if not a.no_closure:
+        out["closure"] = closure_workload(ns)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)
