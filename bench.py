@@ -173,8 +173,12 @@ def closure_workload(ns, seeds=64, snapshots=64, stride=10, epochs=10, members=1
 
 
 def cpu_baseline(n):
-    """Oracle (reference algorithm, project FFT, OpenMP) on this host: one step
-    of the same forced integrating-factor workload at FP64."""
+    """Oracle (reference algorithm, project FFT, OpenMP) on this host at FP64:
+    one step of the same forced integrating-factor workload (the figure that
+    sits beside the GPU value), then one step each of the reference's own
+    classic RK4 (spectral_time_integration.c:34-106, with its redundant
+    transforms) and of the same RK4 with only the transforms it needs, on the
+    same field."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -190,9 +194,21 @@ def cpu_baseline(n):
     o.d.dt = o.cfl()
     o.advance_if(o.project_forcing(f.generate(o.d.dt), 1e-3))
     t = time.perf_counter() - t0
-    return dict(value=n * n / t / 1e9, unit="Gpoint-steps/s", cores=threads, kind="port",
-                sample=f"1 RK4 step of the {n}^2 forced integrating-factor workload, FP64, "
-                       f"oracle with project FFT (not FFTW), {t:.2f} s")
+    out = dict(value=n * n / t / 1e9, unit="Gpoint-steps/s", cores=threads, kind="port",
+               sample=f"1 RK4 step of the {n}^2 forced integrating-factor workload, FP64, "
+                      f"oracle with project FFT (not FFTW), {t:.2f} s")
+    # the reference's step itself (no forcing, drag or integrating factor)
+    o.d.drag = 0.0
+    o.compute_derivatives()
+    for key, minimal in (("reference_rk4_step", False), ("minimal_rk4_step", True)):
+        c0 = o.d.fft_count
+        t0 = time.perf_counter()
+        o.advance_rk4(minimal=minimal)
+        t = time.perf_counter() - t0
+        out[key] = {"value": round(n * n / t / 1e9, 5), "unit": "Gpoint-steps/s",
+                    "seconds": round(t, 2), "ffts": int(o.d.fft_count - c0)}
+    o.close()
+    return out
 
 
 def main():
