@@ -312,6 +312,54 @@ struct CLine {
 // LDS offset of element i under the one-in-sixteen padding
 constexpr int padc(int i) { return i + (i >> 4); }
 
+// Twiddle sources of the compile-time passes.  A radix-R butterfly of pass
+// PASS needs W^(k 2^e), e = 0 .. log2(R) - 1, for k = its index mod Ns.
+// TwTable reads them from the table in global memory.  TwRegs holds W^k of
+// every full-radix pass in registers, loaded once per thread, and forms the
+// powers by repeated squaring: no global load then sits inside those passes,
+// so they never wait for loads issued ahead of the transform (the vector
+// memory counter retires loads in order).  Squaring adds a few ulp to the
+// higher powers (W^4k is three roundings from the table value).  The short
+// closing pass of a length that is not a power of P (several butterflies
+// per thread) still reads the table.
+template <typename T>
+struct TwTable {
+    const cx<T>* __restrict__ tw;
+    template <int L, int PASS>
+    __device__ __forceinline__ cx<T> get(int, int e, int k) const {
+        constexpr int tstride = L / (CLine<L>::stride(PASS) * CLine<L>::radix(PASS));
+        return tw[(k << e) * tstride];
+    }
+};
+
+template <int L, typename T>
+struct TwRegs {
+    const cx<T>* __restrict__ tab;
+    cx<T> w[CLine<L>::passes];  // W^k of each full-radix pass
+    template <int L2, int PASS>
+    __device__ __forceinline__ cx<T> get(int q, int e, int k) const {
+        if constexpr (CLine<L>::radix(PASS) == P) {
+            cx<T> r = w[PASS];
+            for (int i = 0; i < e; ++i) r = cmul(r, r);
+            return r;
+        } else {
+            return TwTable<T>{tab}.template get<L, PASS>(q, e, k);
+        }
+    }
+};
+
+template <int L, int PASS, typename T>
+__device__ __forceinline__ void load_tw_regs(TwRegs<L, T>& t, int j, const cx<T>* __restrict__ tw) {
+    if constexpr (PASS == 0) t.tab = tw;
+    if constexpr (PASS < CLine<L>::passes) {
+        using G = CLine<L>;
+        constexpr int Ns = G::stride(PASS);
+        if constexpr (Ns > 1 && G::radix(PASS) == P)
+            t.w[PASS] = tw[(j & (Ns - 1)) * (L / (Ns * P))];
+        load_tw_regs<L, PASS + 1>(t, j, tw);
+    }
+}
+
 template <int L, typename T>
 __device__ __forceinline__ void lds_read_c(cx<T> (&v)[P], const cx<T>* lds, int j) {
     constexpr int TPL = CLine<L>::TPL;
@@ -338,26 +386,24 @@ __device__ __forceinline__ void lds_write_c(const cx<T> (&v)[P], cx<T>* lds, int
     }
 }
 
-template <int DIR, int L, int PASS, typename T>
-__device__ __forceinline__ void cpass(cx<T> (&v)[P], int j, const cx<T>* __restrict__ tw,
-                                      cx<T>* lds) {
+template <int DIR, int L, int PASS, typename T, typename TW>
+__device__ __forceinline__ void cpass(cx<T> (&v)[P], int j, const TW& tw, cx<T>* lds) {
     using G = CLine<L>;
     constexpr int R = G::radix(PASS);
     constexpr int Ns = G::stride(PASS);
     constexpr bool last = (PASS == G::passes - 1);
     constexpr int Q = P / R;
     constexpr int TPL = G::TPL;
-    constexpr int tstride = L / (Ns * R);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int b = j + q * TPL;
         const int k = b & (Ns - 1);
         if constexpr (Ns > 1) {
             cx<T> w[R];
-            w[1] = tw[k * tstride];
-            if constexpr (R > 2) w[2] = tw[2 * k * tstride];
-            if constexpr (R > 4) w[4] = tw[4 * k * tstride];
-            if constexpr (R > 8) w[8] = tw[8 * k * tstride];
+            w[1] = tw.template get<L, PASS>(q, 0, k);
+            if constexpr (R > 2) w[2] = tw.template get<L, PASS>(q, 1, k);
+            if constexpr (R > 4) w[4] = tw.template get<L, PASS>(q, 2, k);
+            if constexpr (R > 8) w[8] = tw.template get<L, PASS>(q, 3, k);
             if constexpr (R > 2) w[3] = cmul(w[1], w[2]);
             if constexpr (R > 4) {
                 w[5] = cmul(w[1], w[4]);
@@ -396,9 +442,8 @@ __device__ __forceinline__ void cpass(cx<T> (&v)[P], int j, const cx<T>* __restr
     }
 }
 
-template <int DIR, int L, int PASS, typename T>
-__device__ __forceinline__ void cpasses(cx<T> (&v)[P], cx<T>* lds, int j,
-                                        const cx<T>* __restrict__ tw) {
+template <int DIR, int L, int PASS, typename T, typename TW>
+__device__ __forceinline__ void cpasses(cx<T> (&v)[P], cx<T>* lds, int j, const TW& tw) {
     if constexpr (PASS < CLine<L>::passes) {
         if constexpr (PASS > 0) lds_read_c<L>(v, lds, j);
         __syncthreads();
@@ -412,32 +457,37 @@ __device__ __forceinline__ void cpasses(cx<T> (&v)[P], cx<T>* lds, int j,
 template <int DIR, int L, typename T>
 __device__ __forceinline__ void line_fft_c(cx<T> (&v)[P], cx<T>* lds, int j,
                                            const cx<T>* __restrict__ tw) {
+    cpasses<DIR, L, 0>(v, lds, j, TwTable<T>{tw});
+}
+// the same with register-resident twiddles
+template <int DIR, int L, typename T>
+__device__ __forceinline__ void line_fft_c(cx<T> (&v)[P], cx<T>* lds, int j,
+                                           const TwRegs<L, T>& tw) {
     cpasses<DIR, L, 0>(v, lds, j, tw);
 }
 
 // Two independent transforms of compile-time length L run in lock-step: one
 // barrier pair per pass serves both, the twiddles are computed once, and the
 // two dependency chains interleave.  lds1/lds2 are two separate line regions.
-template <int DIR, int L, int PASS, typename T>
-__device__ __forceinline__ void cpass2(cx<T> (&v1)[P], cx<T> (&v2)[P], int j,
-                                       const cx<T>* __restrict__ tw, cx<T>* lds1, cx<T>* lds2) {
+template <int DIR, int L, int PASS, typename T, typename TW>
+__device__ __forceinline__ void cpass2(cx<T> (&v1)[P], cx<T> (&v2)[P], int j, const TW& tw,
+                                       cx<T>* lds1, cx<T>* lds2) {
     using G = CLine<L>;
     constexpr int R = G::radix(PASS);
     constexpr int Ns = G::stride(PASS);
     constexpr bool last = (PASS == G::passes - 1);
     constexpr int Q = P / R;
     constexpr int TPL = G::TPL;
-    constexpr int tstride = L / (Ns * R);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int b = j + q * TPL;
         const int k = b & (Ns - 1);
         if constexpr (Ns > 1) {
             cx<T> w[R];
-            w[1] = tw[k * tstride];
-            if constexpr (R > 2) w[2] = tw[2 * k * tstride];
-            if constexpr (R > 4) w[4] = tw[4 * k * tstride];
-            if constexpr (R > 8) w[8] = tw[8 * k * tstride];
+            w[1] = tw.template get<L, PASS>(q, 0, k);
+            if constexpr (R > 2) w[2] = tw.template get<L, PASS>(q, 1, k);
+            if constexpr (R > 4) w[4] = tw.template get<L, PASS>(q, 2, k);
+            if constexpr (R > 8) w[8] = tw.template get<L, PASS>(q, 3, k);
             if constexpr (R > 2) w[3] = cmul(w[1], w[2]);
             if constexpr (R > 4) {
                 w[5] = cmul(w[1], w[4]);
@@ -490,9 +540,9 @@ __device__ __forceinline__ void cpass2(cx<T> (&v1)[P], cx<T> (&v2)[P], int j,
     }
 }
 
-template <int DIR, int L, int PASS, typename T>
+template <int DIR, int L, int PASS, typename T, typename TW>
 __device__ __forceinline__ void cpasses2(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* lds1,
-                                         cx<T>* lds2, int j, const cx<T>* __restrict__ tw) {
+                                         cx<T>* lds2, int j, const TW& tw) {
     if constexpr (PASS < CLine<L>::passes) {
         if constexpr (PASS > 0) {
             lds_read_c<L>(v1, lds1, j);
@@ -509,6 +559,11 @@ __device__ __forceinline__ void cpasses2(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* 
 template <int DIR, int L, typename T>
 __device__ __forceinline__ void line_fft2_c(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* lds1,
                                             cx<T>* lds2, int j, const cx<T>* __restrict__ tw) {
+    cpasses2<DIR, L, 0>(v1, v2, lds1, lds2, j, TwTable<T>{tw});
+}
+template <int DIR, int L, typename T>
+__device__ __forceinline__ void line_fft2_c(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* lds1,
+                                            cx<T>* lds2, int j, const TwRegs<L, T>& tw) {
     cpasses2<DIR, L, 0>(v1, v2, lds1, lds2, j, tw);
 }
 

@@ -196,6 +196,11 @@ __device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const 
     if constexpr (LC > 0) line_fft_c<DIR, LC>(v, lds, j, tw);
     else line_fft<DIR>(v, lds, j, g, tw);
 }
+template <int DIR, int LC, typename T>
+__device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const Line&,
+                                        const TwRegs<LC, T>& tw) {
+    line_fft_c<DIR, LC>(v, lds, j, tw);
+}
 
 // Kept bands of the 2/3 rule (spectral_initialization.c:46-48) for lines of
 // compile-time length LC: last kept kx of a row of length LC and last kept
@@ -691,9 +696,9 @@ __device__ __forceinline__ void build_two_rows2(cx<T> (&v1)[P], cx<T> (&v2)[P], 
 // Forward x-transform of two real rows packed as v = ra + i rb, then
 // separation of the two half spectra and store of the kept columns into
 // the mixed layout at rows y and y+1.
-template <int LC, typename T>
+template <int LC, typename T, typename TW>
 __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
-                                               const Geo& G, const cx<T>* tw, cx<T>* out, int y,
+                                               const Geo& G, const TW& tw, cx<T>* out, int y,
                                                bool active) {
     const int TPL = LC > 0 ? LC / P : g.TPL;
     const int Ll = LC > 0 ? LC : g.L;
@@ -714,6 +719,63 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
             const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
             out[(size_t)k * G.ny + y] = ja;
             out[(size_t)k * G.ny + y + 1] = jb;
+        }
+    }
+}
+
+// Kept columns of one field at rows y, y + 1 (the two rows of one packed
+// line) for this thread's slots, held in registers from the gather to the
+// LDS staging.  Zero for an inactive line.
+template <typename T>
+struct RowPairRegs {
+    cx<T> a[P], b[P];
+};
+
+template <int LC, typename T>
+__device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const Geo& G,
+                                                const cx<T>* F, int y, bool active) {
+    constexpr int kxm = Band<LC>::kxm;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+        const int kind = slot_kind(s, LC, kxm);
+        const int k = j + s * (LC / P);
+        if (kind == 1 || (kind == 3 && k <= kxm)) {
+            if (active) {
+                const cx<T>* src = F + (size_t)k * G.ny + y;
+                r.a[s] = src[0];
+                r.b[s] = src[1];
+            } else {
+                r.a[s] = mk(T(0), T(0));
+                r.b[s] = mk(T(0), T(0));
+            }
+        }
+    }
+}
+
+// The LDS image of stage_two_rows, from gathered registers.
+template <int LC, typename T>
+__device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const RowPairRegs<T>& r) {
+    constexpr int TPL = LC / P, kxm = Band<LC>::kxm;
+    const cx<T> zero = mk(T(0), T(0));
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+        const int kind = slot_kind(s, LC, kxm);
+        const int k = j + s * TPL;
+        if (kind == 2) continue;  // written by the owner of LC - k
+        if (kind == 0) {
+            lds[pad(k)] = zero;
+            continue;
+        }
+        if (kind == 1 || k <= kxm) {
+            cx<T> a = r.a[s], b = r.b[s];
+            if (k == 0) {
+                a.y = T(0);
+                b.y = T(0);
+            }
+            lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
+            if (k > 0) lds[pad(LC - k)] = mk(a.x + b.y, b.x - a.y);
+        } else if (k < LC - kxm) {
+            lds[pad(k)] = zero;
         }
     }
 }
@@ -783,7 +845,56 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
     T u2max = T(0);
+    if constexpr (paired_lines<T, LC, NT, PR>() && LC >= 2048) {
+        // Long rows.  Twiddles first, then the psi gathers, then the omega gathers,
+        // which stay in flight while the psi fields are transformed: with
+        // the twiddles in registers the transforms issue no global loads
+        // that would have to wait behind the gathers.
+        TwRegs<LC, T> tw;
+        load_tw_regs<LC, 0>(tw, j, a.tw);
+        // (always true: the grid holds no idle workgroup; as a branch it keeps
+        // the compiler's schedule of the loads above within 128 VGPRs)
+        if (blk * lines < G.ny / 2) {
+            RowPairRegs<T> r0, r1, q0, q1;
+            gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
+            gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
+            stage_two_rows_regs<LC>(lds, j, r0);
+            stage_two_rows_regs<LC>(lds2, j, r1);
+            gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
+            gather_two_rows<LC>(q1, j, G, a.mixed[3] + moff, y0, active);
+            __syncthreads();
+            cx<T> v2[P];
+            lds_read_c<LC>(v, lds, j);
+            lds_read_c<LC>(v2, lds2, j);
+            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_x, psi_y
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                T sa = v[s].x * v[s].x + v2[s].x * v2[s].x;  // u = psi_y, v = -psi_x
+                T sb = v[s].y * v[s].y + v2[s].y * v2[s].y;
+                if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max
+                if (sb != sb) sb = T(__builtin_inf());
+                u2max = sa > u2max ? sa : u2max;
+                u2max = sb > u2max ? sb : u2max;
+                X[s] = v[s];
+                Jp[s] = v2[s];
+            }
+            __syncthreads();
+            stage_two_rows_regs<LC>(lds, j, q0);
+            stage_two_rows_regs<LC>(lds2, j, q1);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            lds_read_c<LC>(v2, lds2, j);
+            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // omega_x, omega_y
+#pragma unroll
+            for (int s = 0; s < P; ++s)
+                v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
+            fwd_rows_store<LC>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
+        }
+        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        return;
+    }
     if constexpr (paired_lines<T, LC, NT, PR>()) {
+        // shorter rows (fewer registers to spare): gathers at their use
         cx<T> v2[P];
         build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0] + moff, a.mixed[1] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y
