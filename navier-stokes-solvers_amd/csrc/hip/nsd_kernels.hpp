@@ -918,6 +918,66 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
+    if constexpr (LC >= 2048 && NT <= 512) {
+        // Long rows, one line region (FP64): each field's gather is in flight
+        // while the previous field is transformed (twiddles in registers, as
+        // in the paired path above).  Not for the 1024-thread workgroups of
+        // the longest rows, whose 128-register budget it would overflow.
+        TwRegs<LC, T> tw;
+        load_tw_regs<LC, 0>(tw, j, a.tw);
+        // (always true; as a branch it keeps the schedule of the loads above
+        // within the register budget)
+        if (blk * lines < G.ny / 2) {
+            RowPairRegs<T> r;
+            gather_two_rows<LC>(r, j, G, a.mixed[0] + moff, y0, active);
+            stage_two_rows_regs<LC>(lds, j, r);
+            gather_two_rows<LC>(r, j, G, a.mixed[1] + moff, y0, active);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            line_fft_c<+1, LC>(v, lds, j, tw);  // psi_x
+#pragma unroll
+            for (int s = 0; s < P; ++s) X[s] = v[s];
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+            stage_two_rows_regs<LC>(lds, j, r);
+            gather_two_rows<LC>(r, j, G, a.mixed[2] + moff, y0, active);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            line_fft_c<+1, LC>(v, lds, j, tw);  // psi_y
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                // CFL speed: u = psi_y, v = -psi_x (spectral_core.c:30-31)
+                T sa = X[s].x * X[s].x + v[s].x * v[s].x;
+                T sb = X[s].y * X[s].y + v[s].y * v[s].y;
+                if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max
+                if (sb != sb) sb = T(__builtin_inf());
+                u2max = sa > u2max ? sa : u2max;
+                u2max = sb > u2max ? sb : u2max;
+                Jp[s] = v[s];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+            stage_two_rows_regs<LC>(lds, j, r);
+            gather_two_rows<LC>(r, j, G, a.mixed[3] + moff, y0, active);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            line_fft_c<+1, LC>(v, lds, j, tw);  // omega_x
+#pragma unroll
+            for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+            stage_two_rows_regs<LC>(lds, j, r);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            line_fft_c<+1, LC>(v, lds, j, tw);  // omega_y
+#pragma unroll
+            for (int s = 0; s < P; ++s)
+                v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
+            fwd_rows_store<LC>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
+        }
+        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        return;
+    }
     build_two_rows<LC>(v, lds, j, g, G, a.mixed[0] + moff, y0, active);  // psi_x
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
