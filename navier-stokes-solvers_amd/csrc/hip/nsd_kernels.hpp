@@ -355,8 +355,38 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     cx<T> v[P];
     cx<T> w[P];
 
+    // y-band of the 2/3 mask: compile-time for compile-time column lengths
+    const int kym = LC > 0 ? Band<LC>::kym : G.kymax;
+    const int ncy = LC > 0 ? 2 * Band<LC>::kym + 1 : G.nky_keep;
+    // Long columns of a stage: the state and accumulator values of this
+    // column are loaded ahead of the forward transform, which runs while they
+    // arrive (its twiddles in registers, loaded first, so that it has no
+    // global load of its own to wait behind them).
+    constexpr bool PF = MODE == COL_STAGE && LC >= 2048 && paired_lines<T, LC, NT, PR>();
+    cx<T> pw0[P], pacc[P];
+
     // ---- forward y-transform of the mixed nonlinear field
-    if constexpr (MODE == COL_STAGE || MODE == COL_FWD) {
+    if constexpr (PF) {
+        TwRegs<LC, T> tw;
+        load_tw_regs<LC, 0>(tw, j, a.tw);
+#pragma unroll
+        for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            const int kind = slot_kind(s, LC, Band<LC>::kym);
+            const int jy = j + s * TPL;
+            if (kind == 0 || !has_data) continue;
+            const bool pos = kind == 1 || (kind == 3 && jy <= kym);
+            if (kind == 3 && !pos && jy < Ll - kym) continue;
+            const size_t ci = (size_t)kxi * ncy + (pos ? jy : jy - (Ll - ncy));
+            pw0[s] = state[ci];
+            pacc[s] = a.stage > 0 ? acc[ci] : zero;
+        }
+        fft_any<-1, LC>(v, lds, j, g, tw);
+        // nothing that uses the values loaded above may be scheduled into
+        // the transform (it would wait there for them)
+        __builtin_amdgcn_sched_barrier(0);
+    } else if constexpr (MODE == COL_STAGE || MODE == COL_FWD) {
 #pragma unroll
         for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
         fft_any<-1, LC>(v, lds, j, g, a.tw);
@@ -398,9 +428,6 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const double sfd_alpha = (ph.sfd_tau > 0.0) ? fmin(fmax(dt / ph.sfd_tau, 0.0), 1.0) : 0.0;
     const T sqh = T(sqrt(dt));
 
-    // y-band of the 2/3 mask: compile-time for compile-time column lengths
-    const int kym = LC > 0 ? Band<LC>::kym : G.kymax;
-    const int ncy = LC > 0 ? 2 * Band<LC>::kym + 1 : G.nky_keep;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
         const int kind = LC > 0 ? slot_kind(s, LC, Band<LC>::kym) : 3;
@@ -422,7 +449,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
         cx<T> N = (-norm) * v[s];
         if (src) N = N + src[ci];  // learned subgrid tendency, frozen over the step
-        const cx<T> w0 = state[ci];
+        const cx<T> w0 = PF ? pw0[s] : state[ci];
         const T k2 = kx * kx + ky * ky;
         const T kmag = sqrt(k2);
         cx<T> wn;
@@ -438,13 +465,13 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 acc[ci] = w0 + (h / T(6)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
             } else if (a.stage == 1) {
-                acc[ci] = acc[ci] + (h / T(3)) * ks;
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
             } else if (a.stage == 2) {
-                acc[ci] = acc[ci] + (h / T(3)) * ks;
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * ks;
                 wn = w0 + h * ks;
             } else {
-                wn = acc[ci] + (h / T(6)) * ks;
+                wn = (PF ? pacc[s] : acc[ci]) + (h / T(6)) * ks;
                 if (ph.sfd_on && ph.sfd_tau > 0.0) {  // hyperviscosity.c:152-178
                     const T al = T(sfd_alpha);
                     sfd[ci] = (T(1) - al) * sfd[ci] + al * wn;
@@ -463,13 +490,13 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 acc[ci] = grow(w0, e1) + (h / T(6)) * grow(N, e1);
                 wn = grow(w0 + (T(0.5) * h) * N, e2);
             } else if (a.stage == 1) {
-                acc[ci] = acc[ci] + (h / T(3)) * grow(N, e2);
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e2) + (T(0.5) * h) * N;
             } else if (a.stage == 2) {
-                acc[ci] = acc[ci] + (h / T(3)) * grow(N, e2);
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e1) + h * grow(N, e2);
             } else {
-                wn = acc[ci] + (h / T(6)) * N;
+                wn = (PF ? pacc[s] : acc[ci]) + (h / T(6)) * N;
             }
         }
         if (a.stage == 3) {
@@ -517,6 +544,49 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         // transformed two at a time: (omega_y, omega_x), then w becomes
         // psi_hat = -w/k^2 in place and (psi_y, psi_x)
         cx<T> v2[P];
+        if constexpr (LC >= 2048) {
+            // Long columns, unrolled: the twiddles of the second pair go into
+            // registers before the first pair's stores are issued, so that
+            // its transform does not wait for those stores to retire (the
+            // vector-memory counter retires stores and loads in order).
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                v[s] = mul_i(w[s], ky);
+                v2[s] = mul_i(w[s], kx);
+            }
+            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // omega_y, omega_x
+            TwRegs<LC, T> tw;
+            load_tw_regs<LC, 0>(tw, j, a.tw);
+            if (has_data) {
+                cx<T>* o1 = a.mixed[3] + mixed_off;
+                cx<T>* o2 = a.mixed[2] + mixed_off;
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    o1[mbase + j + s * TPL] = v[s];
+                    o2[mbase + j + s * TPL] = v2[s];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                const T k2 = kx * kx + ky * ky;
+                const cx<T> psi = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                v[s] = mul_i(psi, ky);
+                v2[s] = mul_i(psi, kx);
+            }
+            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_y, psi_x
+            if (has_data) {
+                cx<T>* o1 = a.mixed[1] + mixed_off;
+                cx<T>* o2 = a.mixed[0] + mixed_off;
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    o1[mbase + j + s * TPL] = v[s];
+                    o2[mbase + j + s * TPL] = v2[s];
+                }
+            }
+            return;
+        }
 #pragma unroll 1
         for (int f = 3; f >= 1; f -= 2) {
             if (f == 1) {
