@@ -358,11 +358,11 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     // y-band of the 2/3 mask: compile-time for compile-time column lengths
     const int kym = LC > 0 ? Band<LC>::kym : G.kymax;
     const int ncy = LC > 0 ? 2 * Band<LC>::kym + 1 : G.nky_keep;
-    // Long columns of a stage: the state and accumulator values of this
-    // column are loaded ahead of the forward transform, which runs while they
+    // Paired stage columns of compile-time length: the state and accumulator
+    // values of this column are loaded ahead of the forward transform, which runs while they
     // arrive (its twiddles in registers, loaded first, so that it has no
     // global load of its own to wait behind them).
-    constexpr bool PF = MODE == COL_STAGE && LC >= 2048 && paired_lines<T, LC, NT, PR>();
+    constexpr bool PF = MODE == COL_STAGE && LC >= 64 && paired_lines<T, LC, NT, PR>();
     cx<T> pw0[P], pacc[P];
 
     // ---- forward y-transform of the mixed nonlinear field
@@ -544,8 +544,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         // transformed two at a time: (omega_y, omega_x), then w becomes
         // psi_hat = -w/k^2 in place and (psi_y, psi_x)
         cx<T> v2[P];
-        if constexpr (LC >= 2048) {
-            // Long columns, unrolled: the twiddles of the second pair go into
+        if constexpr (LC >= 64) {
+            // Unrolled: the twiddles of the second pair go into
             // registers before the first pair's stores are issued, so that
             // its transform does not wait for those stores to retire (the
             // vector-memory counter retires stores and loads in order).
@@ -850,6 +850,15 @@ __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const Row
     }
 }
 
+// Paired row stages that issue the omega gathers ahead of the psi
+// transforms: every compile-time length but 1024, whose closing radix-2 pass
+// (four butterflies per thread) pushes that schedule past the register
+// budget (128 FP32 / 256 FP64 VGPRs) into scratch.
+template <int LC>
+constexpr bool row_prefetch() {
+    return LC >= 64 && LC != 1024;
+}
+
 template <typename T, int NT, int MODE, int LC, int PR = -1>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -915,8 +924,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
     T u2max = T(0);
-    if constexpr (paired_lines<T, LC, NT, PR>() && LC >= 2048) {
-        // Long rows.  Twiddles first, then the psi gathers, then the omega gathers,
+    if constexpr (paired_lines<T, LC, NT, PR>() && row_prefetch<LC>()) {
+        // Twiddles first, then the psi gathers, then the omega gathers,
         // which stay in flight while the psi fields are transformed: with
         // the twiddles in registers the transforms issue no global loads
         // that would have to wait behind the gathers.
@@ -964,7 +973,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         return;
     }
     if constexpr (paired_lines<T, LC, NT, PR>()) {
-        // shorter rows (fewer registers to spare): gathers at their use
+        // rows of 1024: gathers at their use
         cx<T> v2[P];
         build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0] + moff, a.mixed[1] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y
