@@ -61,7 +61,7 @@ def kernel_bytes(n, fp32):
 PROBE_BYTES = 1 << 30
 
 
-def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0):
+def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0, graph=True):
     L = ns.load()
     s = workload(ns, n, fp32)
     ctx = s.ctx
@@ -71,7 +71,7 @@ def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0):
         s.check(L.nsd_bandwidth_probe(ctx, PROBE_BYTES, 5, C.byref(ms)))
         probe_gbs = 2 * PROBE_BYTES / (ms.value * 1e-3) / 1e9
     if warmup:
-        s.check(L.nsd_advance_graph(ctx, warmup, 1))
+        s.check(L.nsd_advance_graph(ctx, warmup, 1) if graph else L.nsd_advance(ctx, warmup, 1))
     s.check(L.nsd_synchronize(ctx))
     # timed region: eager launches with per-kernel events
     L.nsd_profile(ctx, 1)
@@ -84,14 +84,16 @@ def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0):
     nst = C.c_int()
     L.nsd_profile_read(ctx, C.byref(row_ms), C.byref(col_ms), C.byref(step_ms), C.byref(nst))
     # the same steps replayed from the captured graph (host clock)
-    t0 = time.perf_counter()
-    s.check(L.nsd_advance_graph(ctx, steps, 1))
-    s.check(L.nsd_synchronize(ctx))
-    graph_wall = time.perf_counter() - t0
+    graph_wall = None
+    if graph:
+        t0 = time.perf_counter()
+        s.check(L.nsd_advance_graph(ctx, steps, 1))
+        s.check(L.nsd_synchronize(ctx))
+        graph_wall = time.perf_counter() - t0
     st = s.sync_host()
     e, z = s.device_diagnostics()
     res = dict(ms_per_step=step_ms.value, wall_ms_per_step=1e3 * wall / steps,
-               graph_ms_per_step=1e3 * graph_wall / steps, row_ms=row_ms.value,
+               graph_ms_per_step=1e3 * graph_wall / steps if graph else None, row_ms=row_ms.value,
                col_ms=col_ms.value, t=s.d.t, dt=s.d.dt, energy=e, enstrophy=z, status=st,
                probe_gbs=probe_gbs, compat_ms_per_step=None)
     if compat:
@@ -225,6 +227,9 @@ def main():
                     help="skip the 256-member mixed-size ensemble")
     ap.add_argument("--no-compat", action="store_true",
                     help="skip timing the PCIe-bound reference entry point ns_spectral_advance_rk4")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches only (no captured-graph warm-up or replay), for tools "
+                         "that cannot follow graph launches")
     ap.add_argument("--probe", action="store_true",
                     help="also time a plain 1 GiB device copy (k_copy) for counter calibration")
     a = ap.parse_args()
@@ -236,8 +241,10 @@ def main():
     import __graft_entry__ as G
     ns = G._import_package()
     n = a.n
-    r32 = time_gpu(ns, n, True, a.steps, a.warmup, a.probe, compat=0 if a.no_compat else 3)
-    r64 = None if a.no_fp64 else time_gpu(ns, n, False, max(a.steps // 2, 1), max(a.warmup // 2, 1))
+    r32 = time_gpu(ns, n, True, a.steps, a.warmup, a.probe, compat=0 if a.no_compat else 3,
+                   graph=not a.no_graph)
+    r64 = None if a.no_fp64 else time_gpu(ns, n, False, max(a.steps // 2, 1), max(a.warmup // 2, 1),
+                                          graph=not a.no_graph)
     value = n * n / (r32["ms_per_step"] * 1e-3) / 1e9
 
     row_b, col_b, M, Cc = kernel_bytes(n, True)
@@ -276,7 +283,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": dom_b},
         "step": {"row_ms": round(r32["row_ms"], 4), "col_ms": round(r32["col_ms"], 4),
-                 "graph_ms_per_step": round(r32["graph_ms_per_step"], 4),
+                 "graph_ms_per_step": (round(r32["graph_ms_per_step"], 4)
+                                       if r32["graph_ms_per_step"] else None),
                  "algorithmic_bytes_per_step": step_bytes,
                  "step_hbm_frac": round(step_bytes / (r32["ms_per_step"] * 1e-3) / 1e9
                                         / HBM_PEAK_GBS, 4),
