@@ -604,6 +604,10 @@ struct NsdContext {
     NsdParams P;
     Phys phys;
     hipStream_t stream = nullptr;
+    // the forcing draw runs beside stage 1 on its own stream (fork after
+    // stage 0, join before stage 2)
+    hipStream_t fstream = nullptr;
+    hipEvent_t fev[2] = {nullptr, nullptr};
     size_t esz;  // bytes per complex element at the working precision
     void* state = nullptr;
     void* acc = nullptr;
@@ -820,8 +824,13 @@ void enqueue_stage(NsdContext* c, int s, int adaptive) {
 }
 
 template <typename T>
-void enqueue_force(NsdContext* c) {
-    if (!c->phys.force_on || c->fA == 0) return;
+bool force_active(const NsdContext* c) {
+    return c->phys.force_on && c->fA != 0;
+}
+
+template <typename T>
+void enqueue_force(NsdContext* c, hipStream_t st) {
+    if (!force_active<T>(c)) return;
     ForceArgs f;
     memset(&f, 0, sizeof(f));
     f.geo = c->G;
@@ -841,7 +850,7 @@ void enqueue_force(NsdContext* c) {
     f.fbox = c->fbox;
     f.sc = c->sc;
     f.m_seed = c->m_seed;
-    hipLaunchKernelGGL(k_force, dim3(1, c->nb), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_force, dim3(1, c->nb), dim3(256), 0, st, f);
 }
 
 // learned subgrid tendency of the state at the start of the step, into the
@@ -888,13 +897,23 @@ void enqueue_closure(NsdContext* c) {
 }
 
 // one RK4 step: stage 0 computes dt (adaptive) before the forcing draw,
-// which needs dt for the colored-noise decay and is consumed on stage 3
+// which needs dt for the colored-noise decay and is consumed on stage 3.
+// The draw (one workgroup) runs on the side stream beside stage 1, which
+// neither reads nor writes what it touches (its box, OU state, work area
+// and the forcing counter).
 template <typename T>
 void enqueue_step(NsdContext* c, int adaptive) {
     if (c->closure) enqueue_closure<T>(c);
+    const bool side = force_active<T>(c);
     for (int s = 0; s < 4; ++s) {
+        if (s == 2 && side) (void)hipStreamWaitEvent(c->stream, c->fev[1], 0);
         enqueue_stage<T>(c, s, adaptive);
-        if (s == 0) enqueue_force<T>(c);
+        if (s == 0 && side) {
+            (void)hipEventRecord(c->fev[0], c->stream);
+            (void)hipStreamWaitEvent(c->fstream, c->fev[0], 0);
+            enqueue_force<T>(c, c->fstream);
+            (void)hipEventRecord(c->fev[1], c->fstream);
+        }
     }
 }
 
@@ -1419,7 +1438,11 @@ NsdContext* nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision
         }
         return false;
     };
-    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return nullptr;
+    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) ||
+        bad(hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking)) ||
+        bad(hipEventCreateWithFlags(&c->fev[0], hipEventDisableTiming)) ||
+        bad(hipEventCreateWithFlags(&c->fev[1], hipEventDisableTiming)))
+        return nullptr;
     const size_t C = C_of(c) * c->esz * members, M = M_of(c) * c->esz * members;
     const size_t S = sizeof(DevScalars) * members;
     if (bad(hipMalloc(&c->state, C)) || bad(hipMalloc(&c->acc, C)) ||
@@ -1480,6 +1503,10 @@ void nsd_destroy(NsdContext* c) {
                     c->cl_m, c->cl_x, c->cl_y, c->cl_src};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->fstream) (void)hipStreamSynchronize(c->fstream);
+    for (auto e : c->fev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->fstream) (void)hipStreamDestroy(c->fstream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -801,6 +801,20 @@ struct RowPairRegs {
     cx<T> a[P], b[P];
 };
 
+// Element i of a mixed field at (workgroup-uniform) base F, addressed as the
+// base plus a 32-bit byte offset: the loads then take the scalar base and one
+// 32-bit offset register per slot.  A mixed field is at most 2731 x 8192
+// FP64 complex values (358 MB), well inside 32 bits.
+template <typename T>
+__device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
+    return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
+                                          i * (unsigned)sizeof(cx<T>));
+}
+template <typename T>
+__device__ __forceinline__ cx<T>* at32(cx<T>* F, unsigned i) {
+    return reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(F) + i * (unsigned)sizeof(cx<T>));
+}
+
 template <int LC, typename T>
 __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const Geo& G,
                                                 const cx<T>* F, int y, bool active) {
@@ -811,7 +825,7 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
         const int k = j + s * (LC / P);
         if (kind == 1 || (kind == 3 && k <= kxm)) {
             if (active) {
-                const cx<T>* src = F + (size_t)k * G.ny + y;
+                const cx<T>* src = at32(F, (unsigned)(k * G.ny + y));
                 r.a[s] = src[0];
                 r.b[s] = src[1];
             } else {
