@@ -10,9 +10,11 @@
 //   forward:        A[p][(tap, ci)] = X[p + off(tap)][ci],  B = W[n][tap][ci]
 //   input gradient: A[p][(tap, co)] = dZ[p - off(tap)][co], B = W[co][tap][n]
 //   weight grad.:   dW[co][(tap, ci)] = sum_p dZ[p][co] X[p + off(tap)][ci]
-// A wave computes 16 pixels x 16 channels per MFMA; the B operand (at most
-// 288 x 32 bf16) sits in LDS, transposed so each lane's 4 k-values are one
-// 8-byte read.
+// A wave computes 16 pixels x 16 channels per MFMA (v_mfma_f32_16x16x32_bf16,
+// depth 32: one 3x3 tap of 32 channels); the B operand (at most 288 x 32
+// bf16) sits in LDS, transposed so each lane's 8 k-values are one 16-byte
+// read, and each lane's A values are one 16-byte load of 8 channels.  A workgroup stages B once and walks TILES tiles of 64
+// pixels, so the staging is paid once per 64 x TILES pixels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +30,8 @@
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -42,16 +46,19 @@ __device__ __forceinline__ bf16 f2bf(float f) {  // round to nearest even
     return (bf16)(u >> 16);
 }
 
-// GELU, tanh form: g(z) = z/2 (1 + tanh(c (z + a z^3))), c = sqrt(2/pi), a = 0.044715
-__device__ __forceinline__ float gelu(float z) {
+// GELU, tanh form: g(z) = z/2 (1 + tanh(u)), u = c (z + a z^3), c = sqrt(2/pi),
+// a = 0.044715.  Since (1 + tanh u)/2 = sigma(2u) = 1/(1 + e^(-2u)), it is
+// evaluated as z sigma(2u) with the hardware exponential and reciprocal
+// (a few ulp of FP32; the results are stored as bf16).  g'(z) = s + z s (1 - s)
+// 2 c (1 + 3 a z^2) with s = sigma(2u).
+__device__ __forceinline__ float sig2u(float z) {
     const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    return 0.5f * z * (1.f + tanhf(u));
+    return __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
 }
+__device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
 __device__ __forceinline__ float gelu_d(float z) {
-    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    const float t = tanhf(u);
-    return 0.5f * (1.f + t) +
-           0.5f * z * (1.f - t * t) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+    const float sg = sig2u(z);
+    return sg + z * sg * (1.f - sg) * (2.f * 0.7978845608028654f) * (1.f + 3.f * 0.044715f * z * z);
 }
 
 // pixel p (row-major in an n x n grid, n a power of two) shifted by tap's
@@ -69,7 +76,7 @@ struct ConvArgs {
     const bf16* in;     // A source: [batch][npix][KC]
     const bf16* w;      // layer weights, bf16 [cout][9][cin]
     const float* bias;  // [NN] (forward)
-    bf16* z;            // forward: pre-activations [batch][npix][NN]
+    bf16* z;            // forward: pre-activations [batch][npix][NN] (null: not kept)
     bf16* act;          // forward: GELU(z)
     float* y;           // last layer: [batch][npix]
     float yscale;       // last layer: output multiplied by this
@@ -77,80 +84,96 @@ struct ConvArgs {
     bf16* dzprev;       // input gradient: dL/dz of the layer below
 };
 
-// KC = channels of the A source, NN = output channels of this GEMM
-template <int KC, int NN, int MODE>
+// KC = channels of the A source, NN = output channels of this GEMM, TILES =
+// 64-pixel tiles per workgroup (npix must be a multiple of 64 TILES)
+template <int KC, int NN, int MODE, int TILES>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
-    constexpr int KP = (9 * KC + 15) / 16 * 16;  // GEMM depth, padded
+    constexpr int KP = (9 * KC + 31) / 32 * 32;  // GEMM depth, padded to the MFMA depth
     constexpr int NP = (NN + 15) / 16 * 16;      // GEMM width, padded
     __shared__ __align__(16) bf16 Bt[NP * KP];   // B transposed: Bt[n][k]
     const int tid = threadIdx.x;
     for (int e = tid; e < NP * KP; e += 256) {
         const int nn = e / KP, k = e % KP;
-        float v = 0.f;
+        bf16 v = 0;
         if (nn < NN && k < 9 * KC) {
             const int tap = k / KC, kc = k % KC;
-            v = MODE == CONV_BWD_DATA ? bf2f(a.w[(kc * 9 + tap) * NN + nn])
-                                      : bf2f(a.w[(nn * 9 + tap) * KC + kc]);
+            v = MODE == CONV_BWD_DATA ? a.w[(kc * 9 + tap) * NN + nn] : a.w[(nn * 9 + tap) * KC + kc];
         }
-        Bt[e] = f2bf(v);
+        Bt[e] = v;
     }
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, kq = lane >> 4;
     const size_t boff = (size_t)blockIdx.y * a.npix;
-    const int p0 = (blockIdx.x * 4 + wave) * 16;
-    const int p = p0 + m;
     const int s = MODE == CONV_BWD_DATA ? -1 : 1;
-    f4 acc[NP / 16];
+#pragma unroll 1
+    for (int t = 0; t < TILES; ++t) {
+        const int p0 = ((blockIdx.x * TILES + t) * 4 + wave) * 16;
+        const int p = p0 + m;
+        f4 acc[NP / 16];
 #pragma unroll
-    for (int nb = 0; nb < NP / 16; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < NP / 16; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int c = 0; c < KP / 16; ++c) {
-        const int k0 = c * 16 + 4 * kq;
-        s4 av;
-        if constexpr (KC % 16 == 0) {
-            const int tap = k0 / KC, kc = k0 % KC;
-            const bf16* src = a.in + (boff + shift(p, tap, s, a.lg, a.n)) * KC + kc;
-            av = *reinterpret_cast<const s4*>(src);
-        } else {
+        for (int c = 0; c < KP / 32; ++c) {
+            const int k0 = c * 32 + 8 * kq;  // A lane l: row l % 16, k = 8 (l / 16) + i
+            s8 av;
+            if constexpr (KC % 32 == 0) {
+                const int tap = k0 / KC, kc = k0 % KC;
+                const bf16* src = a.in + (boff + shift(p, tap, s, a.lg, a.n)) * KC + kc;
+                av = *reinterpret_cast<const s8*>(src);
+            } else {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int k = k0 + i;
-                short v = 0;
-                if (k < 9 * KC) {
-                    const int tap = k / KC, kc = k % KC;
-                    v = (short)a.in[(boff + shift(p, tap, s, a.lg, a.n)) * KC + kc];
+                for (int i = 0; i < 8; ++i) {
+                    const int k = k0 + i;
+                    short v = 0;
+                    if (k < 9 * KC) {
+                        const int tap = k / KC, kc = k % KC;
+                        v = (short)a.in[(boff + shift(p, tap, s, a.lg, a.n)) * KC + kc];
+                    }
+                    av[i] = v;
                 }
-                av[i] = v;
+            }
+#pragma unroll
+            for (int nb = 0; nb < NP / 16; ++nb) {
+                const s8 bv = *reinterpret_cast<const s8*>(&Bt[(nb * 16 + m) * KP + k0]);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av),
+                                                                  __builtin_bit_cast(bf8, bv),
+                                                                  acc[nb], 0, 0, 0);
             }
         }
+        // D[row = 4 kq + i][col = lane & 15]
 #pragma unroll
         for (int nb = 0; nb < NP / 16; ++nb) {
-            const s4 bv = *reinterpret_cast<const s4*>(&Bt[(nb * 16 + m) * KP + k0]);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[nb], 0, 0, 0);
-        }
-    }
-    // D[row = 4 kq + i][col = lane & 15]
+            const int nn = nb * 16 + m;
 #pragma unroll
-    for (int nb = 0; nb < NP / 16; ++nb) {
-        const int nn = nb * 16 + m;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const size_t q = boff + p0 + 4 * kq + i;
-            const float v = acc[nb][i];
-            if constexpr (MODE == CONV_FWD_GELU) {
-                if (nn < NN) {
-                    const float zz = v + a.bias[nn];
-                    a.z[q * NN + nn] = f2bf(zz);
-                    a.act[q * NN + nn] = f2bf(gelu(zz));
+            for (int i = 0; i < 4; ++i) {
+                const size_t q = boff + p0 + 4 * kq + i;
+                const float v = acc[nb][i];
+                if constexpr (MODE == CONV_FWD_GELU) {
+                    if (nn < NN) {
+                        const float zz = v + a.bias[nn];
+                        if (a.z) a.z[q * NN + nn] = f2bf(zz);
+                        a.act[q * NN + nn] = f2bf(gelu(zz));
+                    }
+                } else if constexpr (MODE == CONV_FWD_LAST) {
+                    if (nn == 0) a.y[q] = (v + a.bias[0]) * a.yscale;
+                } else {
+                    if (nn < NN) a.dzprev[q * NN + nn] = f2bf(v * gelu_d(bf2f(a.zprev[q * NN + nn])));
                 }
-            } else if constexpr (MODE == CONV_FWD_LAST) {
-                if (nn == 0) a.y[q] = (v + a.bias[0]) * a.yscale;
-            } else {
-                if (nn < NN) a.dzprev[q * NN + nn] = f2bf(v * gelu_d(bf2f(a.zprev[q * NN + nn])));
             }
         }
-    }
+    }  // tiles
+}
+
+// grid and launch of k_conv: 8 tiles per workgroup when the grid allows it
+template <int KC, int NN, int MODE>
+void launch_conv(const ConvArgs& a, int batch, hipStream_t st) {
+    if (a.npix % (64 * 8) == 0)
+        hipLaunchKernelGGL((k_conv<KC, NN, MODE, 8>), dim3(a.npix / (64 * 8), batch), dim3(256), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL((k_conv<KC, NN, MODE, 1>), dim3(a.npix / 64, batch), dim3(256), 0, st,
+                           a);
 }
 
 // dW[co][(tap, ci)] += sum_q dZ[q][co] X[q + off(tap)][ci] over this
@@ -359,9 +382,10 @@ unsigned long long splitmix(unsigned long long& s) {
     return z ^ (z >> 31);
 }
 
-// the four layers on `st`, output (times yscale) into y
-int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStream_t st) {
-    const dim3 grid(c->npix / 64, batch);
+// the four layers on `st`, output (times yscale) into y; keep_z keeps the
+// pre-activations for a backward pass
+int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStream_t st,
+                      bool keep_z) {
     ConvArgs a;
     memset(&a, 0, sizeof(a));
     a.n = c->n;
@@ -373,22 +397,21 @@ int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStrea
         a.w = c->wb + c->bwoff[l];
         a.bias = c->prm + c->boff[l];
         if (l < 3) {
-            a.z = c->z[l];
+            a.z = keep_z ? c->z[l] : nullptr;
             a.act = c->act[l];
         } else {
             a.y = y;
         }
-        if (l == 0)
-            hipLaunchKernelGGL((k_conv<1, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0, st, a);
-        else if (l < 3)
-            hipLaunchKernelGGL((k_conv<kHidden, kHidden, CONV_FWD_GELU>), grid, dim3(256), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_conv<kHidden, 1, CONV_FWD_LAST>), grid, dim3(256), 0, st, a);
+        if (l == 0) launch_conv<1, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else if (l < 3) launch_conv<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else launch_conv<kHidden, 1, CONV_FWD_LAST>(a, batch, st);
     }
     return ck(c, hipGetLastError(), "closure forward") ? 3 : 0;
 }
 
-int launch_forward(NsdClosure* c, int batch) { return launch_forward_to(c, batch, c->y, 1.f, c->stream); }
+int launch_forward(NsdClosure* c, int batch) {
+    return launch_forward_to(c, batch, c->y, 1.f, c->stream, true);
+}
 
 int adam_step(NsdClosure* c, float lr) {
     const float b1 = 0.9f, b2 = 0.999f;
@@ -416,7 +439,6 @@ int launch_backward(NsdClosure* c, int batch) {
     a.n = c->n;
     a.lg = c->lg;
     a.npix = c->npix;
-    const dim3 grid(c->npix / 64, batch);
     for (int l = kLayers - 1; l >= 0; --l) {
         const bf16* dzl = l == 3 ? c->dy : c->dz[l];
         d.x = l == 0 ? c->x0 : c->act[l - 1];
@@ -434,11 +456,8 @@ int launch_backward(NsdClosure* c, int batch) {
         a.w = c->wb + c->bwoff[l];
         a.zprev = c->z[l - 1];
         a.dzprev = c->dz[l - 1];
-        if (l == 3)
-            hipLaunchKernelGGL((k_conv<1, kHidden, CONV_BWD_DATA>), grid, dim3(256), 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((k_conv<kHidden, kHidden, CONV_BWD_DATA>), grid, dim3(256), 0,
-                               c->stream, a);
+        if (l == 3) launch_conv<1, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
+        else launch_conv<kHidden, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
     }
     return ck(c, hipGetLastError(), "closure backward") ? 3 : 0;
 }
@@ -711,7 +730,7 @@ int nsd_closure_eval_batch(NsdClosure* c, const float* x, float* y, int batch, v
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (size_t)batch * c->npix, x,
                        c->in_scale, c->x0);
-    return launch_forward_to(c, batch, y, c->out_scale, st);
+    return launch_forward_to(c, batch, y, c->out_scale, st, false);
 }
 
 int nsd_closure_eval_device(NsdClosure* c, const float* x, float* y, void* stream) {
