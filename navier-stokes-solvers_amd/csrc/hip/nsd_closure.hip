@@ -136,29 +136,37 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 #pragma unroll
             for (int nb = 0; nb < NP / 16; ++nb) {
                 const s8 bv = *reinterpret_cast<const s8*>(&Bt[(nb * 16 + m) * KP + k0]);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av),
-                                                                  __builtin_bit_cast(bf8, bv),
-                                                                  acc[nb], 0, 0, 0);
+                if constexpr (MODE == CONV_FWD_LAST)  // D[channel][pixel]
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, bv), __builtin_bit_cast(bf8, av), acc[nb], 0, 0, 0);
+                else  // D[pixel][channel]
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv), acc[nb], 0, 0, 0);
             }
         }
-        // D[row = 4 kq + i][col = lane & 15]
+        if constexpr (MODE == CONV_FWD_LAST) {
+            // D[row = channel 4 kq + i][col = pixel p0 + (lane & 15)]: channel 0
+            // sits in lanes 0..15, one pixel each
+            if (kq == 0) a.y[boff + p] = (acc[0][0] + a.bias[0]) * a.yscale;
+        } else {
+            // D[row = pixel p0 + 4 kq + i][col = channel nb 16 + (lane & 15)]
 #pragma unroll
-        for (int nb = 0; nb < NP / 16; ++nb) {
-            const int nn = nb * 16 + m;
+            for (int nb = 0; nb < NP / 16; ++nb) {
+                const int nn = nb * 16 + m;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const size_t q = boff + p0 + 4 * kq + i;
-                const float v = acc[nb][i];
-                if constexpr (MODE == CONV_FWD_GELU) {
-                    if (nn < NN) {
-                        const float zz = v + a.bias[nn];
-                        if (a.z) a.z[q * NN + nn] = f2bf(zz);
-                        a.act[q * NN + nn] = f2bf(gelu(zz));
+                for (int i = 0; i < 4; ++i) {
+                    const size_t q = boff + p0 + 4 * kq + i;
+                    const float v = acc[nb][i];
+                    if constexpr (MODE == CONV_FWD_GELU) {
+                        if (nn < NN) {
+                            const float zz = v + a.bias[nn];
+                            if (a.z) a.z[q * NN + nn] = f2bf(zz);
+                            a.act[q * NN + nn] = f2bf(gelu(zz));
+                        }
+                    } else {
+                        if (nn < NN)
+                            a.dzprev[q * NN + nn] = f2bf(v * gelu_d(bf2f(a.zprev[q * NN + nn])));
                     }
-                } else if constexpr (MODE == CONV_FWD_LAST) {
-                    if (nn == 0) a.y[q] = (v + a.bias[0]) * a.yscale;
-                } else {
-                    if (nn < NN) a.dzprev[q * NN + nn] = f2bf(v * gelu_d(bf2f(a.zprev[q * NN + nn])));
                 }
             }
         }
