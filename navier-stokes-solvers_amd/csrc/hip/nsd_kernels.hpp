@@ -359,9 +359,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const int kym = LC > 0 ? Band<LC>::kym : G.kymax;
     const int ncy = LC > 0 ? 2 * Band<LC>::kym + 1 : G.nky_keep;
     // Paired stage columns of compile-time length: the state and accumulator
-    // values of this column are loaded ahead of the forward transform, which runs while they
-    // arrive (its twiddles in registers, loaded first, so that it has no
-    // global load of its own to wait behind them).
+    // values of this column are loaded ahead of the forward transform, which
+    // runs while they arrive (its twiddles in registers, loaded first, so that
+    // it has no global load of its own to wait behind them).
     constexpr bool PF = MODE == COL_STAGE && LC >= 64 && paired_lines<T, LC, NT, PR>();
     cx<T> pw0[P], pacc[P];
 
@@ -763,6 +763,20 @@ __device__ __forceinline__ void build_two_rows2(cx<T> (&v1)[P], cx<T> (&v2)[P], 
     lds_read_c<LC>(v2, lds2, j);
 }
 
+// Element i of a mixed field at (workgroup-uniform) base F, addressed as the
+// base plus a 32-bit byte offset: the loads then take the scalar base and one
+// 32-bit offset register per slot.  A mixed field is at most 2731 x 8192
+// FP64 complex values (358 MB), well inside 32 bits.
+template <typename T>
+__device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
+    return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
+                                          i * (unsigned)sizeof(cx<T>));
+}
+template <typename T>
+__device__ __forceinline__ cx<T>* at32(cx<T>* F, unsigned i) {
+    return reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(F) + i * (unsigned)sizeof(cx<T>));
+}
+
 // Forward x-transform of two real rows packed as v = ra + i rb, then
 // separation of the two half spectra and store of the kept columns into
 // the mixed layout at rows y and y+1.
@@ -787,8 +801,9 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
             const cx<T> zm = lds[pad((Ll - k) & (Ll - 1))];
             const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
             const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
-            out[(size_t)k * G.ny + y] = ja;
-            out[(size_t)k * G.ny + y + 1] = jb;
+            cx<T>* dst = at32(out, (unsigned)(k * G.ny + y));
+            dst[0] = ja;
+            dst[1] = jb;
         }
     }
 }
@@ -800,20 +815,6 @@ template <typename T>
 struct RowPairRegs {
     cx<T> a[P], b[P];
 };
-
-// Element i of a mixed field at (workgroup-uniform) base F, addressed as the
-// base plus a 32-bit byte offset: the loads then take the scalar base and one
-// 32-bit offset register per slot.  A mixed field is at most 2731 x 8192
-// FP64 complex values (358 MB), well inside 32 bits.
-template <typename T>
-__device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
-    return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
-                                          i * (unsigned)sizeof(cx<T>));
-}
-template <typename T>
-__device__ __forceinline__ cx<T>* at32(cx<T>* F, unsigned i) {
-    return reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(F) + i * (unsigned)sizeof(cx<T>));
-}
 
 template <int LC, typename T>
 __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const Geo& G,
