@@ -314,6 +314,17 @@ __device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
 template <typename T>
 constexpr int min_waves() { return sizeof(T) == 4 ? 4 : 2; }
 
+// Twiddle source of a kernel's compile-time transforms: registers (REG) or
+// the table in global memory.
+template <bool REG, int L, typename T>
+struct TwSource {
+    using type = const cx<T>*;
+};
+template <int L, typename T>
+struct TwSource<true, L, T> {
+    using type = TwRegs<L, T>;
+};
+
 template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -362,7 +373,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     // values of this column are loaded ahead of the forward transform, which
     // runs while they arrive (its twiddles in registers, loaded first, so that
     // it has no global load of its own to wait behind them).
-    constexpr bool PF = MODE == COL_STAGE && LC >= 64 && paired_lines<T, LC, NT, PR>();
+    constexpr bool PF = MODE == COL_STAGE && LC >= 64 &&
+                        (paired_lines<T, LC, NT, PR>() || (sizeof(T) == 8 && NT <= 512));
     cx<T> pw0[P], pacc[P];
 
     // ---- forward y-transform of the mixed nonlinear field
@@ -616,7 +628,13 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         }
     } else {
         // outputs 0 psi_x, 1 psi_y, 2 omega_x, 3 omega_y (spectral_core.c:62-99);
-        // the omega-based pair first, then w becomes psi_hat = -w/k^2 in place
+        // the omega-based pair first, then w becomes psi_hat = -w/k^2 in place.
+        // FP64 stage columns take their twiddles from registers, so that no
+        // transform waits for the previous field's stores to retire.
+        constexpr bool REG = PF && sizeof(T) == 8;
+        typename TwSource<REG, (LC > 0 ? LC : 16), T>::type tws;
+        if constexpr (REG) load_tw_regs<LC, 0>(tws, j, a.tw);
+        else tws = a.tw;
 #pragma unroll 1
         for (int f = 3; f >= 0; --f) {
             if (f == 1) {
@@ -632,7 +650,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 const T ky = T(band_of(G, j + s * TPL)) * kys;
                 v[s] = mul_i(w[s], (f & 1) ? ky : kx);
             }
-            fft_any<+1, LC>(v, lds, j, g, a.tw);
+            fft_any<+1, LC>(v, lds, j, g, tws);
             if (has_data) {
                 cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
