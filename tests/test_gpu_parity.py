@@ -358,3 +358,33 @@ def test_conservation_suite(ns):
         assert np.abs(div).max() < 1e-12
     E1, Z1 = s.energy(), s.enstrophy()
     assert 0.01 < E1 / E0 < 1.0 and 0 < Z1 < Z0
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_forced_eager_steps_equal_graph_replay(ns, fp32):
+    """The forcing draw runs on a side stream beside stage 1 (fork after
+    stage 0, join before stage 2).  Eager launches (nsd_advance) and the
+    captured one-step graph (nsd_advance_graph) must give the same bits."""
+    import ctypes as C
+    n, steps = 128, 30
+    runs = []
+    for graph in (False, True):
+        s = solver(ns, n, Re=1e4)
+        s.init(taylor_green=False)
+        s.hv_set(enabled=1, coefficient=1e-16 * (np.sqrt(2) * n / 2) ** 8, order=8, cutoff_ratio=0.0)
+        s.set_precision(fp32)
+        s.set_scheme(True)
+        s.set_drag(0.1)
+        s.set_forcing(4.0, 6.0, 1e-3, seed=42, colored=True, correlation_time=0.3)
+        s.random_phase(7, k_peak=10.0)
+        s.sync_device()
+        L = s.L
+        adv = L.nsd_advance_graph if graph else L.nsd_advance
+        assert adv(s.ctx, steps, 1) == 0
+        assert L.nsd_synchronize(s.ctx) == 0
+        assert s.sync_host() == 0
+        runs.append((s.spec("omega_hat").copy(), s.d.t))
+        s.close()
+    (w_eager, t_eager), (w_graph, t_graph) = runs
+    assert t_eager == t_graph
+    assert np.array_equal(w_eager, w_graph)
