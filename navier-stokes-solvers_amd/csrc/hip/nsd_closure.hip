@@ -39,12 +39,8 @@ constexpr int kHidden = 32;  // channels of the three hidden layers
 constexpr int kLayers = 4;
 
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
-__device__ __forceinline__ bf16 f2bf(float f) {  // round to nearest even
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (bf16)(u >> 16);
-}
+// round to nearest even: gfx950's v_cvt_pk_bf16_f32 (NaN stays NaN)
+__device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 
 // GELU, tanh form: g(z) = z/2 (1 + tanh(u)), u = c (z + a z^3), c = sqrt(2/pi),
 // a = 0.044715.  Since (1 + tanh u)/2 = sigma(2u) = 1/(1 + e^(-2u)), it is
