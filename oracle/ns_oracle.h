@@ -27,6 +27,12 @@
  *                    oracle/_ref/ and against committed golden vectors
  *   energy/enstrophy Taylor-Green closed forms E = 0.25 e^{-4 nu t}, Z = 0.5 e^{-4 nu t}
  *   spectrum         shell sums vs numpy binning of the same modes
+ *
+ * Parity unpinned: the complete classic RK4 step on a generic field (J != 0)
+ * as one composite.  No compiled reference step exists to compare it with
+ * (the reference's spectral path needs FFTW, absent here); it is pinned only
+ * through its parts above and through the test that the minimal step equals
+ * the reference's own 40-transform call sequence.
  */
 #ifndef NS_ORACLE_H
 #define NS_ORACLE_H
