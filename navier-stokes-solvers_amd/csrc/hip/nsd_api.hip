@@ -1019,7 +1019,10 @@ int op_random_phase(NsdContext* c, unsigned long long seed, const unsigned long 
     hipLaunchKernelGGL(k_random_phase<T>, grid, dim3(256), 0, c->stream, c->G, seed, seeds,
                        k_peak, (cx<T>*)c->state);
     hipLaunchKernelGGL(k_clear_red, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb);
-    hipLaunchKernelGGL(k_state_energy<T>, grid, dim3(256), 0, c->stream, c->G,
+    // One workgroup per member: its single atomic add onto zero makes the
+    // normalising energy, and so the initial field, the same on every call
+    // (several workgroups would add their partial sums in arrival order).
+    hipLaunchKernelGGL(k_state_energy<T>, dim3(1, c->nb), dim3(256), 0, c->stream, c->G,
                        (const cx<T>*)c->state, c->sc);
     hipLaunchKernelGGL(k_scale_state<T>, grid, dim3(256), 0, c->stream, C_of(c),
                        (cx<T>*)c->state, (const DevScalars*)c->sc, energy);

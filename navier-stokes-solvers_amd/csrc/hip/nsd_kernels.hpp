@@ -884,12 +884,14 @@ __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const Row
 }
 
 // Paired row stages that issue the omega gathers ahead of the psi
-// transforms: every compile-time length but 1024, whose closing radix-2 pass
-// (four butterflies per thread) pushes that schedule past the register
-// budget (128 FP32 / 256 FP64 VGPRs) into scratch.
-template <int LC>
+// transforms: every compile-time length at FP32, and at FP64 all but 1024,
+// whose closing radix-2 pass (four butterflies per thread) pushes that
+// schedule past the 256-register budget into 44 bytes of scratch.  FP32 rows
+// of 1024 spill 8-20 bytes with it and still run faster than without it
+// (256-member mixed-size ensemble: 87.0k -> 90.4k member-steps/s).
+template <typename T, int LC>
 constexpr bool row_prefetch() {
-    return LC >= 64 && LC != 1024;
+    return LC >= 64 && (LC != 1024 || sizeof(T) == 4);
 }
 
 template <typename T, int NT, int MODE, int LC, int PR = -1>
@@ -957,7 +959,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
     T u2max = T(0);
-    if constexpr (paired_lines<T, LC, NT, PR>() && row_prefetch<LC>()) {
+    if constexpr (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()) {
         // Twiddles first, then the psi gathers, then the omega gathers,
         // which stay in flight while the psi fields are transformed: with
         // the twiddles in registers the transforms issue no global loads
@@ -1006,7 +1008,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         return;
     }
     if constexpr (paired_lines<T, LC, NT, PR>()) {
-        // rows of 1024: gathers at their use
+        // FP64 rows of 1024: gathers at their use
         cx<T> v2[P];
         build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0] + moff, a.mixed[1] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y

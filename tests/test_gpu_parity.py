@@ -377,14 +377,17 @@ def test_forced_eager_steps_equal_graph_replay(ns, fp32):
         s.set_drag(0.1)
         s.set_forcing(4.0, 6.0, 1e-3, seed=42, colored=True, correlation_time=0.3)
         s.random_phase(7, k_peak=10.0)
+        w0 = s.spec("omega_hat").copy()
         s.sync_device()
         L = s.L
         adv = L.nsd_advance_graph if graph else L.nsd_advance
         assert adv(s.ctx, steps, 1) == 0
         assert L.nsd_synchronize(s.ctx) == 0
         assert s.sync_host() == 0
-        runs.append((s.spec("omega_hat").copy(), s.d.t))
+        runs.append((w0, s.spec("omega_hat").copy(), s.d.t))
         s.close()
-    (w_eager, t_eager), (w_graph, t_graph) = runs
+    (w0_eager, w_eager, t_eager), (w0_graph, w_graph, t_graph) = runs
+    # the seeded initial field itself is reproducible bit for bit
+    assert np.array_equal(w0_eager, w0_graph)
     assert t_eager == t_graph
     assert np.array_equal(w_eager, w_graph)
