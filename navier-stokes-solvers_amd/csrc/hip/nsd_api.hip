@@ -270,15 +270,6 @@ __global__ void k_d2f(size_t n, const double* a, float* out) {
         out[e] = (float)a[e];
 }
 
-// zero the reduction slots of every member's scalars (a kernel, not a
-// memset, so that it may sit inside a captured graph)
-__global__ void k_clear_red(DevScalars* sc, int nb) {
-    for (int m = threadIdx.x; m < nb; m += blockDim.x) {
-        for (int i = 0; i < 8; ++i) sc[m].red[i] = 0.0;
-        for (int i = 0; i < 4; ++i) sc[m].redmax[i] = 0ull;
-    }
-}
-
 // time, dt and forcing counter of every member
 __global__ void k_set_scalars(DevScalars* sc, int nb, int which, double t, double dt,
                               unsigned long long fcount) {
@@ -293,12 +284,13 @@ __global__ void k_set_scalars(DevScalars* sc, int nb, int which, double t, doubl
 
 // Parseval sums over the compact state: E = 1/2 sum_full |w|^2/k^2,
 // Z = 1/2 sum_full |w|^2 (half-plane weights 1 at kx = 0, else 2).
+// This thread's share of both sums for member blockIdx.y.
 template <typename T>
-__global__ void k_state_energy(Geo G, const cx<T>* st, DevScalars* sc) {
+__device__ void state_energy_terms(const Geo& G, const cx<T>* st, double& e, double& z) {
     const size_t n = (size_t)G.nkx_keep * G.nky_keep;
     st += blockIdx.y * n;  // member blockIdx.y of a batch
-    sc += blockIdx.y;
-    double e = 0.0, z = 0.0;
+    e = 0.0;
+    z = 0.0;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
         const int kxi = (int)(q / G.nky_keep), cy = (int)(q % G.nky_keep);
@@ -310,9 +302,54 @@ __global__ void k_state_energy(Geo G, const cx<T>* st, DevScalars* sc) {
         z += 0.5 * wt * a2;
         if (k2 > 1e-12) e += 0.5 * wt * a2 / k2;
     }
+}
+
+// Sum of a (blockDim.x <= 256)-thread workgroup in a fixed order, valid in
+// thread 0.
+__device__ double block_sum_fixed(double v, double* red_lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red_lds[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red_lds[i];
+    return s;
+}
+
+// Parseval sums in a fixed order, so that they are the same bits on every
+// call: each workgroup stores its partials, part[2 (member nparts + block)
+// + {0, 1}], and k_sum_parts adds them in index order.
+template <typename T>
+__global__ void k_state_energy_part(Geo G, const cx<T>* st, double* part) {
+    double e, z;
+    state_energy_terms(G, st, e, z);
     __shared__ double red[4];
-    block_sum_to(e, &sc->red[0], red);
-    block_sum_to(z, &sc->red[1], red);
+    e = block_sum_fixed(e, red);
+    z = block_sum_fixed(z, red);
+    if (threadIdx.x == 0) {
+        const size_t o = 2 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
+        part[o] = e;
+        part[o + 1] = z;
+    }
+}
+
+// One workgroup per member (blockIdx.x): sc[m].red[0..1] = sums of its partials.
+__global__ void k_sum_parts(const double* part, int nparts, DevScalars* sc) {
+    part += 2 * (size_t)blockIdx.x * nparts;
+    double e = 0.0, z = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        e += part[2 * i];
+        z += part[2 * i + 1];
+    }
+    __shared__ double red[4];
+    e = block_sum_fixed(e, red);
+    z = block_sum_fixed(z, red);
+    if (threadIdx.x == 0) {
+        sc[blockIdx.x].red[0] = e;
+        sc[blockIdx.x].red[1] = z;
+    }
 }
 
 __device__ __forceinline__ unsigned long long splitmix64_at(unsigned long long seed,
@@ -593,6 +630,7 @@ struct NsdContext {
     int nb = 1;                             // batch members (one field each)
     int vrow = 0, vcol = 0;                 // launch variants of the step kernels
     int maxlds = 65536;                     // LDS bytes per workgroup on this device
+    int ncu = 0;                            // compute units on this device
     double* m_inv_re = nullptr;             // per-member 1/Re (batches), device
     unsigned long long* m_seed = nullptr;   // per-member forcing seeds (batches), device
     unsigned long long* m_ic = nullptr;     // per-member initial-field seeds, device
@@ -812,6 +850,8 @@ void enqueue_stage(NsdContext* c, int s, int adaptive) {
     RowArgs<T> r = row_args<T>(c);
     r.mode = ROW_STAGE;
     r.track_umax = (s == 0) ? 1 : 0;
+    r.stagger_lo = c->ncu;  // second resident workgroup of each CU
+    r.stagger_hi = 2 * c->ncu;
     ColArgs<T> a = col_args<T>(c);
     a.mode = COL_STAGE;
     a.stage = s;
@@ -1012,18 +1052,27 @@ int op_sfd_upload(NsdContext* c, const double* sfd) {
     return sync(c);
 }
 
+// E and Z of every member into sc[m].red[0..1], summed in a fixed order (the
+// same bits on every call).  The workgroup partials go to the J buffer, which
+// holds nothing between steps: 2 doubles per workgroup of 256 kept modes, and
+// the buffer has room for at least one per kept mode.
+template <typename T>
+void enqueue_state_energy(NsdContext* c) {
+    const dim3 grid(grid_for(C_of(c)), c->nb);
+    hipLaunchKernelGGL(k_state_energy_part<T>, grid, dim3(256), 0, c->stream, c->G,
+                       (const cx<T>*)c->state, (double*)c->nl);
+    hipLaunchKernelGGL(k_sum_parts, dim3(c->nb), dim3(256), 0, c->stream, (const double*)c->nl,
+                       (int)grid.x, c->sc);
+}
+
 template <typename T>
 int op_random_phase(NsdContext* c, unsigned long long seed, const unsigned long long* seeds,
                     double k_peak, double energy) {
     const dim3 grid(grid_for(C_of(c)), c->nb);
     hipLaunchKernelGGL(k_random_phase<T>, grid, dim3(256), 0, c->stream, c->G, seed, seeds,
                        k_peak, (cx<T>*)c->state);
-    hipLaunchKernelGGL(k_clear_red, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb);
-    // One workgroup per member: its single atomic add onto zero makes the
-    // normalising energy, and so the initial field, the same on every call
-    // (several workgroups would add their partial sums in arrival order).
-    hipLaunchKernelGGL(k_state_energy<T>, dim3(1, c->nb), dim3(256), 0, c->stream, c->G,
-                       (const cx<T>*)c->state, c->sc);
+    // summed in a fixed order: one seed gives the same field bits on every call
+    enqueue_state_energy<T>(c);
     hipLaunchKernelGGL(k_scale_state<T>, grid, dim3(256), 0, c->stream, C_of(c),
                        (cx<T>*)c->state, (const DevScalars*)c->sc, energy);
     if (op_prime<T>(c)) return ST_MEM;
@@ -1113,9 +1162,7 @@ int op_state_derived(NsdContext* c, double* psi_hat, double* u_hat, double* v_ha
 // E and Z of every member: out[2 m], out[2 m + 1]
 template <typename T>
 int op_state_energy(NsdContext* c, double* out) {
-    hipLaunchKernelGGL(k_clear_red, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb);
-    hipLaunchKernelGGL(k_state_energy<T>, dim3(grid_for(C_of(c)), c->nb), dim3(256), 0, c->stream,
-                       c->G, (const cx<T>*)c->state, c->sc);
+    enqueue_state_energy<T>(c);
     std::vector<DevScalars> h(c->nb);
     if (lastk(c) || down(c, h.data(), c->sc, sizeof(DevScalars) * c->nb) || sync(c)) return ST_MEM;
     for (int m = 0; m < c->nb; ++m) {
@@ -1476,7 +1523,8 @@ NsdContext* nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision
     }
     int dev = 0, maxlds = 0;
     if (bad(hipGetDevice(&dev)) ||
-        bad(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)))
+        bad(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) ||
+        bad(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev)))
         return nullptr;
     const size_t need = std::max(line_lds_bytes(nx, c->esz), line_lds_bytes(ny, c->esz));
     if (need > (size_t)maxlds) {

@@ -162,6 +162,43 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst, 
     }
 }
 
+// Running maximum of squared speeds u^2 + v^2.  Every value is a sum of
+// squares (+0, positive, +inf) or a NaN; for those the unsigned order of the
+// FP32 bit patterns is the numeric order with every NaN above +inf, so one
+// integer max per value does the NaN test and the compare-select at once.  A
+// NaN reads back as +inf: a diverged field still wins the max (and drives the
+// CFL dt to its floor).  FP64 has no 64-bit integer max, so it compares.
+template <typename T>
+struct SpeedMax {
+    T m = T(0);
+    __device__ __forceinline__ void add(T s) {
+        if (s != s) s = T(__builtin_inf());
+        m = s > m ? s : m;
+    }
+    __device__ __forceinline__ T value() const { return m; }
+};
+template <>
+struct SpeedMax<float> {
+    unsigned m = 0u;
+    __device__ __forceinline__ void add(float s) { m = max(m, __float_as_uint(s)); }
+    __device__ __forceinline__ float value() const {
+        const float f = __uint_as_float(m);
+        return f != f ? __builtin_inff() : f;
+    }
+};
+
+// Speeds of two rows transformed together: a holds psi_x, b holds psi_y of
+// rows y0 (real parts) and y0 + 1 (imaginary parts); u = psi_y, v = -psi_x
+// (spectral_core.c:30-31).
+template <typename T>
+__device__ __forceinline__ void add_speeds(SpeedMax<T>& um, const cx<T>* a, const cx<T>* b) {
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+        um.add(a[s].x * a[s].x + b[s].x * b[s].x);
+        um.add(a[s].y * a[s].y + b[s].y * b[s].y);
+    }
+}
+
 __device__ __forceinline__ void block_sum_to(double v, double* dst, double* red_lds) {
     v = wave_sum(v);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -681,6 +718,8 @@ struct RowArgs {
     size_t bm;             // batches: mixed-field stride of member blockIdx.y
     size_t br;             // batches: physical-field stride (ROW_C2R out, ROW_R2C in_t)
     int nbat;              // members along grid y for the transform modes (0 = 1)
+    int stagger_lo;        // ROW_STAGE: first-round workgroups [lo, hi) start late
+    int stagger_hi;
 };
 
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
@@ -785,6 +824,13 @@ __device__ __forceinline__ void build_two_rows2(cx<T> (&v1)[P], cx<T> (&v2)[P], 
 // base plus a 32-bit byte offset: the loads then take the scalar base and one
 // 32-bit offset register per slot.  A mixed field is at most 2731 x 8192
 // FP64 complex values (358 MB), well inside 32 bits.
+// Element (k, y) of a mixed field (column k, row y): k ny + y through the
+// full-rate 24-bit multiply (k and ny are below 2^24; the 32-bit multiply
+// issues at a quarter of the rate).
+__device__ __forceinline__ unsigned col_elem(int k, int ny, int y) {
+    return __umul24((unsigned)k, (unsigned)ny) + (unsigned)y;
+}
+
 template <typename T>
 __device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
     return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
@@ -819,7 +865,7 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
             const cx<T> zm = lds[pad((Ll - k) & (Ll - 1))];
             const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
             const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
-            cx<T>* dst = at32(out, (unsigned)(k * G.ny + y));
+            cx<T>* dst = at32(out, col_elem(k, G.ny, y));
             dst[0] = ja;
             dst[1] = jb;
         }
@@ -844,7 +890,7 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
         const int k = j + s * (LC / P);
         if (kind == 1 || (kind == 3 && k <= kxm)) {
             if (active) {
-                const cx<T>* src = at32(F, (unsigned)(k * G.ny + y));
+                const cx<T>* src = at32(F, col_elem(k, G.ny, y));
                 r.a[s] = src[0];
                 r.b[s] = src[1];
             } else {
@@ -910,6 +956,20 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
+    // Staggered start.  The dispatcher gives every compute unit one workgroup
+    // before it gives any a second, so blocks [ncu, 2 ncu) of member 0 are the
+    // second residents; they start about 16k cycles late.  The two workgroups
+    // of a CU then reach their gather phases at different times, each loading
+    // under the other's transforms, and keep that offset for the whole launch
+    // (a freed slot is refilled at once).  At 4096 (FP32, two resident
+    // workgroups per CU) this takes a launch from 93.1 to 91.0 us; half or
+    // 1.5 times the delay gains less.
+    if constexpr (MODE == ROW_STAGE && LC == 4096 && sizeof(T) == 4) {
+        if (blockIdx.y == 0 && (int)blockIdx.x >= a.stagger_lo && (int)blockIdx.x < a.stagger_hi) {
+            __builtin_amdgcn_s_sleep(127);
+            __builtin_amdgcn_s_sleep(127);
+        }
+    }
     cx<T> v[P];
     // member blockIdx.y of a batch (ROW_STAGE only)
     // member blockIdx.y of a batch (mixed fields, physical fields)
@@ -958,7 +1018,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     // in the order psi_x, psi_y, omega_x, omega_y, so that only two packed
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
-    T u2max = T(0);
+    SpeedMax<T> um;
     if constexpr (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()) {
         // Twiddles first, then the psi gathers, then the omega gathers,
         // which stay in flight while the psi fields are transformed: with
@@ -981,14 +1041,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
             lds_read_c<LC>(v, lds, j);
             lds_read_c<LC>(v2, lds2, j);
             line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_x, psi_y
+            if (a.track_umax) add_speeds(um, v, v2);
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                T sa = v[s].x * v[s].x + v2[s].x * v2[s].x;  // u = psi_y, v = -psi_x
-                T sb = v[s].y * v[s].y + v2[s].y * v2[s].y;
-                if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max
-                if (sb != sb) sb = T(__builtin_inf());
-                u2max = sa > u2max ? sa : u2max;
-                u2max = sb > u2max ? sb : u2max;
                 X[s] = v[s];
                 Jp[s] = v2[s];
             }
@@ -1004,7 +1059,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
                 v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
             fwd_rows_store<LC>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
         }
-        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
     if constexpr (paired_lines<T, LC, NT, PR>()) {
@@ -1012,14 +1067,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         cx<T> v2[P];
         build_two_rows2<LC>(v, v2, lds, lds2, j, g, G, a.mixed[0] + moff, a.mixed[1] + moff, y0, active);
         line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // psi_x, psi_y
+        if (a.track_umax) add_speeds(um, v, v2);
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            T sa = v[s].x * v[s].x + v2[s].x * v2[s].x;  // u = psi_y, v = -psi_x
-            T sb = v[s].y * v[s].y + v2[s].y * v2[s].y;
-            if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max
-            if (sb != sb) sb = T(__builtin_inf());
-            u2max = sa > u2max ? sa : u2max;
-            u2max = sb > u2max ? sb : u2max;
             X[s] = v[s];
             Jp[s] = v2[s];
         }
@@ -1029,7 +1079,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         for (int s = 0; s < P; ++s)
             v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
         fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
-        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
     if constexpr (LC >= 2048 && NT <= 512) {
@@ -1058,17 +1108,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
             __syncthreads();
             lds_read_c<LC>(v, lds, j);
             line_fft_c<+1, LC>(v, lds, j, tw);  // psi_y
+            if (a.track_umax) add_speeds(um, X, v);
 #pragma unroll
-            for (int s = 0; s < P; ++s) {
-                // CFL speed: u = psi_y, v = -psi_x (spectral_core.c:30-31)
-                T sa = X[s].x * X[s].x + v[s].x * v[s].x;
-                T sb = X[s].y * X[s].y + v[s].y * v[s].y;
-                if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max
-                if (sb != sb) sb = T(__builtin_inf());
-                u2max = sa > u2max ? sa : u2max;
-                u2max = sb > u2max ? sb : u2max;
-                Jp[s] = v[s];
-            }
+            for (int s = 0; s < P; ++s) Jp[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
             __syncthreads();
             stage_two_rows_regs<LC>(lds, j, r);
@@ -1089,7 +1131,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
                 v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
             fwd_rows_store<LC>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
         }
-        if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
     build_two_rows<LC>(v, lds, j, g, G, a.mixed[0] + moff, y0, active);  // psi_x
@@ -1098,17 +1140,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     for (int s = 0; s < P; ++s) X[s] = v[s];
     build_two_rows<LC>(v, lds, j, g, G, a.mixed[1] + moff, y0, active);  // psi_y
     fft_any<+1, LC>(v, lds, j, g, a.tw);
+    if (a.track_umax) add_speeds(um, X, v);
 #pragma unroll
-    for (int s = 0; s < P; ++s) {
-        // CFL speed: u = psi_y, v = -psi_x (spectral_core.c:30-31)
-        T sa = X[s].x * X[s].x + v[s].x * v[s].x;
-        T sb = X[s].y * X[s].y + v[s].y * v[s].y;
-        if (sa != sa) sa = T(__builtin_inf());  // NaN must win the max: divergence check
-        if (sb != sb) sb = T(__builtin_inf());
-        u2max = sa > u2max ? sa : u2max;
-        u2max = sb > u2max ? sb : u2max;
-        Jp[s] = v[s];  // psi_y, folded with omega_x next
-    }
+    for (int s = 0; s < P; ++s) Jp[s] = v[s];  // psi_y, folded with omega_x next
     build_two_rows<LC>(v, lds, j, g, G, a.mixed[2] + moff, y0, active);  // omega_x
     fft_any<+1, LC>(v, lds, j, g, a.tw);
 #pragma unroll
@@ -1119,7 +1153,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     for (int s = 0; s < P; ++s)
         v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
     fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
-    if (a.track_umax) block_max_to(active ? double(u2max) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+    if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
 }
 
 }  // namespace nsd
