@@ -101,18 +101,56 @@ __device__ __forceinline__ double sfd_weight(const Phys& p, double kmag) {
     return p.sfd_strength * powi(s, p.hv_order);
 }
 
-// the same weights evaluated at the working precision of the stage kernels
+// x^n (n >= 0) by repeated squaring: log2(n) steps for a runtime n
 template <typename T>
-__device__ __forceinline__ T hv_weight_t(const Phys& p, T kmag) {
-    if (!p.hv_on || kmag <= T(p.hv_kc)) return T(0);
-    return T(p.hv_coeff) * powi((kmag - T(p.hv_kc)) * T(p.hv_invbw), p.hv_order);
+__device__ __forceinline__ T powi_sq(T x, int n) {
+    T r = T(1);
+    for (; n > 0; n >>= 1) {
+        if (n & 1) r *= x;
+        x *= x;
+    }
+    return r;
+}
+
+// The same weights evaluated at the working precision of the stage kernels,
+// from k^2.  With the usual cutoff kc = 0 and an even order 2m the
+// hyperviscous weight is c (k^2 / bw^2)^m, which needs no square root.
+template <typename T>
+__device__ __forceinline__ T hv_weight_t(const Phys& p, T k2) {
+    if (!p.hv_on) return T(0);
+    if (p.hv_kc == 0.0 && (p.hv_order & 1) == 0)
+        return T(p.hv_coeff) * powi_sq(k2 * T(p.hv_invbw * p.hv_invbw), p.hv_order >> 1);
+    const T kmag = sqrt(k2);
+    if (kmag <= T(p.hv_kc)) return T(0);
+    return T(p.hv_coeff) * powi_sq((kmag - T(p.hv_kc)) * T(p.hv_invbw), p.hv_order);
 }
 template <typename T>
-__device__ __forceinline__ T sfd_weight_t(const Phys& p, T kmag) {
-    if (!p.sfd_on || p.hv_invbw <= 0.0 || kmag <= T(p.hv_kc)) return T(0);
+__device__ __forceinline__ T sfd_weight_t(const Phys& p, T k2) {
+    if (!p.sfd_on || p.hv_invbw <= 0.0) return T(0);
+    const T kmag = sqrt(k2);
+    if (kmag <= T(p.hv_kc)) return T(0);
     T s = (kmag - T(p.hv_kc)) * T(p.hv_invbw);
     s = s < T(0) ? T(0) : (s > T(1) ? T(1) : s);
-    return T(p.sfd_strength) * powi(s, p.hv_order);
+    return T(p.sfd_strength) * powi_sq(s, p.hv_order);
+}
+
+// expm1(x) for x <= 0 (integrating-factor exponents L h / 2 with L <= 0).
+// FP32: the Taylor series to x^7 for |x| < 0.35, whose truncation is below
+// 0.35^7 / 8! = 1.6e-8 of |x|, and exp(x) - 1 from the hardware exp below
+// that, where |expm1(x)| > 0.29 bounds the cancellation: 2.6 ulp at most in
+// an FP32 emulation over [-100, 0] (the library expm1f: ~1 ulp, at twice the
+// instructions).  FP64 keeps the library function.
+template <typename T>
+__device__ __forceinline__ T expm1_nonpos(T x) {
+    if constexpr (sizeof(T) == 4) {
+        const float p =
+            x * (1.f + x * (0.5f + x * (1.f / 6 + x * (1.f / 24 + x * (1.f / 120 +
+                 x * (1.f / 720 + x * (1.f / 5040)))))));
+        const float e = __expf(x) - 1.f;
+        return x > -0.35f ? p : e;
+    } else {
+        return expm1(x);
+    }
 }
 
 // spectral_time_integration.c:22-28 with MIN_DT/MAX_DT of common_types.h:32-33
@@ -500,15 +538,14 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         if (src) N = N + src[ci];  // learned subgrid tendency, frozen over the step
         const cx<T> w0 = PF ? pw0[s] : state[ci];
         const T k2 = kx * kx + ky * ky;
-        const T kmag = sqrt(k2);
         cx<T> wn;
         if constexpr (SCHEME == 0) {
             // classic RK4 (spectral_time_integration.c:50-100) with
             // hyperviscosity_apply (hyperviscosity.c:104-150) and drag
             const cx<T> ws = (a.stage == 0) ? w0 : stagebuf[ci];
-            const T lin = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, kmag));
+            const T lin = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, k2));
             cx<T> ks = N + lin * ws;
-            const T sw = sfd_weight_t<T>(ph, kmag);
+            const T sw = sfd_weight_t<T>(ph, k2);
             if (sw > T(0)) ks = ks - sw * (ws - sfd[ci]);
             if (a.stage == 0) {
                 acc[ci] = w0 + (h / T(6)) * ks;
@@ -532,8 +569,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             // carried as E - 1 (expm1): near k = 0 the per-step decay L h is
             // ~1e-5, which 1 + (E - 1) keeps to full relative precision while
             // E itself, rounded to FP32, would lose ~0.3 % of it every step.
-            const T L = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, kmag));
-            const T e2 = expm1(L * (T(0.5) * h));  // E2 - 1, E2 = exp(L h / 2)
+            const T L = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, k2));
+            const T e2 = expm1_nonpos(L * (T(0.5) * h));  // E2 - 1, E2 = exp(L h / 2)
             const T e1 = e2 * (T(2) + e2);          // E - 1,  E = E2^2
             if (a.stage == 0) {
                 acc[ci] = grow(w0, e1) + (h / T(6)) * grow(N, e1);
