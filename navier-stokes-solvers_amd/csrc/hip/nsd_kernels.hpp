@@ -101,6 +101,15 @@ __device__ __forceinline__ double sfd_weight(const Phys& p, double kmag) {
     return p.sfd_strength * powi(s, p.hv_order);
 }
 
+// -1/k^2 of a nonzero wavenumber, for psi_hat = -omega_hat / k^2
+// (spectral_core.c:17-20): FP32 through the hardware reciprocal (1 ulp, where
+// a correctly rounded division takes about ten instructions), FP64 divides.
+template <typename T>
+__device__ __forceinline__ T neg_inv_k2(T k2) {
+    if constexpr (sizeof(T) == 4) return -__builtin_amdgcn_rcpf(k2);
+    else return T(-1) / k2;
+}
+
 // x^n (n >= 0) by repeated squaring: log2(n) steps for a runtime n
 template <typename T>
 __device__ __forceinline__ T powi_sq(T x, int n) {
@@ -373,7 +382,7 @@ __device__ __forceinline__ cx<T> apply_op(int op, cx<T> a, cx<T> b, T kx, T ky, 
     case OP_DY1: return mul_i(b, ky);
     default: break;
     }
-    const cx<T> psi = k2 > T(1e-12) ? (T(-1) / k2) * a : mk(T(0), T(0));
+    const cx<T> psi = k2 > T(1e-12) ? neg_inv_k2(k2) * a : mk(T(0), T(0));
     if (op == OP_PSI0) return psi;
     if (op == OP_U0) return mul_i(psi, ky);
     return mul_i(psi, -kx);  // OP_V0
@@ -657,7 +666,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             for (int s = 0; s < P; ++s) {
                 const T ky = T(band_of(G, j + s * TPL)) * kys;
                 const T k2 = kx * kx + ky * ky;
-                const cx<T> psi = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                const cx<T> psi = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
                 v[s] = mul_i(psi, ky);
                 v2[s] = mul_i(psi, kx);
             }
@@ -680,7 +689,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 for (int s = 0; s < P; ++s) {
                     const T ky = T(band_of(G, j + s * TPL)) * kys;
                     const T k2 = kx * kx + ky * ky;
-                    w[s] = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                    w[s] = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
                 }
             }
 #pragma unroll
@@ -716,7 +725,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
                 for (int s = 0; s < P; ++s) {
                     const T ky = T(band_of(G, j + s * TPL)) * kys;
                     const T k2 = kx * kx + ky * ky;
-                    w[s] = k2 > T(1e-12) ? (T(-1) / k2) * w[s] : zero;
+                    w[s] = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
                 }
             }
 #pragma unroll
