@@ -4,15 +4,19 @@ Workload (the 4096^2 forced-turbulence case the headline metric is quoted on):
   2-D periodic [0, 2 pi]^2, N = 4096, nu = 1e-4 (Re = 1e4), del^8
   hyperviscosity nu4 = 1e-16 applied as an integrating factor, linear drag
   0.1, white-in-time forcing on the shell 4 <= |k| <= 6 at injection rate
-  1e-3 (counter SplitMix64, seed 42), adaptive dt at CFL 0.5, classical RK4
-  stages in FP32 (FP64 measured beside it).  Initial vorticity: random phases
-  with E(k) ~ k^4 exp(-(k/10)^2), unit energy, seed 42 (synthetic data).
+  1e-3 (counter SplitMix64, seed 42), adaptive dt at CFL 0.5, integrating-
+  factor RK4 stages in FP32 (FP64 measured beside it).  Initial vorticity:
+  random phases with E(k) ~ k^4 exp(-(k/10)^2), unit energy, seed 42
+  (synthetic data).
 
 A step is one full RK4 step (4 x {row kernel, column kernel} + the forcing
-draw) over the whole grid, state resident in HBM.  The timed region launches
-the kernels eagerly with a HIP event between every two kernels on the
-solver's stream; the per-kernel averages give the roofline of the dominant
-kernel.  Prints one JSON line.
+draw) over the whole grid, state resident in HBM.  The timed region runs K
+steps the way ns_spectral_advance_n does, one captured step graph replayed K
+times (eager launches with --no-graph), host clock, the solver's stream
+synchronized on both sides.  The same K steps are then launched eagerly, and
+K more with a HIP event around every stage kernel on the solver's stream:
+the per-kernel averages give the roofline of the dominant kernel.  Prints one
+JSON line.
 """
 import argparse
 import ctypes as C
@@ -73,12 +77,21 @@ def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0, graph=True):
     if warmup:
         s.check(L.nsd_advance_graph(ctx, warmup, 1) if graph else L.nsd_advance(ctx, warmup, 1))
     s.check(L.nsd_synchronize(ctx))
-    # timed region: K eager steps, the solver's stream synchronized on both
-    # sides, host clock (nothing else on the stream)
+    # timed region: K steps through the path of ns_spectral_advance_n (one
+    # captured step graph replayed K times; eager launches with graph=False),
+    # the solver's stream synchronized on both sides, host clock (nothing else
+    # on the stream)
     t0 = time.perf_counter()
-    s.check(L.nsd_advance(ctx, steps, 1))
+    s.check(L.nsd_advance_graph(ctx, steps, 1) if graph else L.nsd_advance(ctx, steps, 1))
     s.check(L.nsd_synchronize(ctx))
     wall = time.perf_counter() - t0
+    # the same number of steps launched eagerly, kernel by kernel (host clock)
+    eager_wall = wall
+    if graph:
+        t0 = time.perf_counter()
+        s.check(L.nsd_advance(ctx, steps, 1))
+        s.check(L.nsd_synchronize(ctx))
+        eager_wall = time.perf_counter() - t0
     # the next K steps again, with HIP events on the solver's stream around
     # every stage kernel: the per-kernel durations of the roofline (the
     # events themselves add a few microseconds per kernel boundary)
@@ -89,17 +102,11 @@ def time_gpu(ns, n, fp32, steps, warmup, probe=False, compat=0, graph=True):
     row_ms, col_ms, step_ms = C.c_double(), C.c_double(), C.c_double()
     nst = C.c_int()
     L.nsd_profile_read(ctx, C.byref(row_ms), C.byref(col_ms), C.byref(step_ms), C.byref(nst))
-    # the same steps replayed from the captured graph (host clock)
-    graph_wall = None
-    if graph:
-        t0 = time.perf_counter()
-        s.check(L.nsd_advance_graph(ctx, steps, 1))
-        s.check(L.nsd_synchronize(ctx))
-        graph_wall = time.perf_counter() - t0
     st = s.sync_host()
     e, z = s.device_diagnostics()
     res = dict(ms_per_step=1e3 * wall / steps, events_ms_per_step=step_ms.value,
-               graph_ms_per_step=1e3 * graph_wall / steps if graph else None, row_ms=row_ms.value,
+               launch="graph replay" if graph else "eager",
+               eager_ms_per_step=1e3 * eager_wall / steps, row_ms=row_ms.value,
                col_ms=col_ms.value, t=s.d.t, dt=s.d.dt, energy=e, enstrophy=z, status=st,
                probe_gbs=probe_gbs, compat_ms_per_step=None)
     if compat:
@@ -223,7 +230,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    # about 70 ms of load at 4096^2: after 10 warm-up steps the timed steps
+    # still run ~2 % slow while the clocks ramp up
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -288,10 +297,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": dom_b},
-        "step": {"row_ms": round(r32["row_ms"], 4), "col_ms": round(r32["col_ms"], 4),
+        "step": {"launch": r32["launch"],
+                 "row_ms": round(r32["row_ms"], 4), "col_ms": round(r32["col_ms"], 4),
+                 "eager_ms_per_step": round(r32["eager_ms_per_step"], 4),
                  "events_ms_per_step": round(r32["events_ms_per_step"], 4),
-                 "graph_ms_per_step": (round(r32["graph_ms_per_step"], 4)
-                                       if r32["graph_ms_per_step"] else None),
                  "algorithmic_bytes_per_step": step_bytes,
                  "step_hbm_frac": round(step_bytes / (r32["ms_per_step"] * 1e-3) / 1e9
                                         / HBM_PEAK_GBS, 4),
@@ -305,7 +314,8 @@ def main():
         out["copy_probe_gbs"] = round(r32["probe_gbs"], 1)
     if r64:
         out["fp64"] = {"value": round(n * n / (r64["ms_per_step"] * 1e-3) / 1e9, 3),
-                       "ms_per_step": round(r64["ms_per_step"], 4),
+                       "ms_per_step": round(r64["ms_per_step"], 4), "launch": r64["launch"],
+                       "eager_ms_per_step": round(r64["eager_ms_per_step"], 4),
                        "row_ms": round(r64["row_ms"], 4), "col_ms": round(r64["col_ms"], 4)}
     if not a.no_ensemble:
         out["ensemble"] = time_ensemble(ns, max(a.steps // 2, 5), 3)
