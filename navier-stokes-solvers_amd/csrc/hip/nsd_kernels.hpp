@@ -101,6 +101,15 @@ __device__ __forceinline__ double sfd_weight(const Phys& p, double kmag) {
     return p.sfd_strength * powi(s, p.hv_order);
 }
 
+// band_of for a column of compile-time length LC (= ny): with TPL = LC / P
+// the sign of the band of slot s is then known at compile time for all but
+// one slot.  LC = 0 reads ny from the geometry.
+template <int LC>
+__device__ __forceinline__ int col_band(const Geo& g, int j) {
+    if constexpr (LC > 0) return j <= LC / 2 ? j : j - LC;
+    else return band_of(g, j);
+}
+
 // -1/k^2 of a nonzero wavenumber, for psi_hat = -omega_hat / k^2
 // (spectral_core.c:17-20): FP32 through the hardware reciprocal (1 ulp, where
 // a correctly rounded division takes about ten instructions), FP64 divides.
@@ -620,7 +629,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
 #pragma unroll
             for (int s = 0; s < P; ++s) {
                 const int jy = j + s * TPL;
-                const T ky = T(band_of(G, jy)) * kys;
+                const T ky = T(col_band<LC>(G, jy)) * kys;
                 const T k2 = kx * kx + ky * ky;
                 cx<T> b = zero;
                 if (a.in1_ref && has_data && kept(G, kxi, jy))
@@ -646,7 +655,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             // vector-memory counter retires stores and loads in order).
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                 v[s] = mul_i(w[s], ky);
                 v2[s] = mul_i(w[s], kx);
             }
@@ -664,7 +673,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                 const T k2 = kx * kx + ky * ky;
                 const cx<T> psi = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
                 v[s] = mul_i(psi, ky);
@@ -687,14 +696,14 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             if (f == 1) {
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
-                    const T ky = T(band_of(G, j + s * TPL)) * kys;
+                    const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                     const T k2 = kx * kx + ky * ky;
                     w[s] = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
                 }
             }
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                 v[s] = mul_i(w[s], ky);
                 v2[s] = mul_i(w[s], kx);
             }
@@ -723,14 +732,14 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             if (f == 1) {
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
-                    const T ky = T(band_of(G, j + s * TPL)) * kys;
+                    const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                     const T k2 = kx * kx + ky * ky;
                     w[s] = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
                 }
             }
 #pragma unroll
             for (int s = 0; s < P; ++s) {
-                const T ky = T(band_of(G, j + s * TPL)) * kys;
+                const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                 v[s] = mul_i(w[s], (f & 1) ? ky : kx);
             }
             fft_any<+1, LC>(v, lds, j, g, tws);
