@@ -180,6 +180,141 @@ void launch_conv(const ConvArgs& a, int batch, hipStream_t st) {
                            a);
 }
 
+// Forward layers from an LDS-staged input patch.  A workgroup of 512
+// threads computes a tile of 8 rows x 64 pixels of one member: it stages the
+// tile's input with its one-pixel periodic halo (10 x 66 pixels, all KC
+// channels) in LDS with coalesced 16-byte loads issued together, holds its
+// part of the transposed weights in registers, and then each wave (one tile
+// row, four blocks of 16 pixels) reads every tap's A operand from LDS.
+// k_conv reloads each pixel's neighbourhood from global memory for each tap,
+// a chain of dependent loads per 16-pixel tile.  Same GEMM, lane layouts and
+// epilogue as k_conv; n must be a multiple of 64.
+constexpr int kTileRows = 8, kTileCols = 64;
+
+template <int KC, int NN, int MODE>
+__global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
+    static_assert(MODE == CONV_FWD_GELU || MODE == CONV_FWD_LAST, "forward layers only");
+    static_assert(kTileRows * 64 == 512, "one wave per tile row");
+    constexpr int KP = (9 * KC + 31) / 32 * 32;
+    constexpr int NP = (NN + 15) / 16 * 16;
+    constexpr int PR = kTileRows + 2, PC = kTileCols + 2;
+    __shared__ __align__(16) bf16 Bt[NP * KP];
+    __shared__ __align__(16) bf16 Xs[PR * PC * KC];
+    const int tid = threadIdx.x;
+    const int tiles_x = a.n / kTileCols;
+    const int r0 = (blockIdx.x / tiles_x) * kTileRows, c0 = (blockIdx.x % tiles_x) * kTileCols;
+    const size_t boff = (size_t)blockIdx.y * a.npix;
+    const int nm = a.n - 1;
+    // input patch: Xs[(r PC + c) KC + ch] = in[(r0 + r - 1, c0 + c - 1) mod n][ch]
+    if constexpr (KC % 8 == 0) {
+        constexpr int CH = KC / 8;  // 16-byte pieces per pixel
+#pragma unroll 4
+        for (int e = tid; e < PR * PC * CH; e += 512) {
+            const int pix = e / CH, q = e - pix * CH;
+            const int r = pix / PC, c = pix - r * PC;
+            const int gy = (r0 + r - 1) & nm, gx = (c0 + c - 1) & nm;
+            const s8 v = *reinterpret_cast<const s8*>(
+                a.in + (boff + ((size_t)gy << a.lg) + gx) * KC + 8 * q);
+            *reinterpret_cast<s8*>(&Xs[pix * KC + 8 * q]) = v;
+        }
+    } else {
+        for (int e = tid; e < PR * PC * KC; e += 512) {
+            const int pix = e / KC, q = e - pix * KC;
+            const int r = pix / PC, c = pix - r * PC;
+            const int gy = (r0 + r - 1) & nm, gx = (c0 + c - 1) & nm;
+            Xs[e] = a.in[(boff + ((size_t)gy << a.lg) + gx) * KC + q];
+        }
+    }
+    for (int e = tid; e < NP * KP; e += 512) {
+        const int nn = e / KP, k = e % KP;
+        bf16 v = 0;
+        if (nn < NN && k < 9 * KC) {
+            const int tap = k / KC, kc = k % KC;
+            v = a.w[(nn * 9 + tap) * KC + kc];
+        }
+        Bt[e] = v;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    // this lane's B operands for the whole tile
+    s8 bv[KP / 32][NP / 16];
+#pragma unroll
+    for (int c = 0; c < KP / 32; ++c)
+#pragma unroll
+        for (int nb = 0; nb < NP / 16; ++nb)
+            bv[c][nb] = *reinterpret_cast<const s8*>(&Bt[(nb * 16 + m) * KP + c * 32 + 8 * kq]);
+    const int lr = wave;  // tile row of this wave
+#pragma unroll 1
+    for (int blk = 0; blk < kTileCols / 16; ++blk) {
+        const int lc = blk * 16;
+        f4 acc[NP / 16];
+#pragma unroll
+        for (int nb = 0; nb < NP / 16; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < KP / 32; ++c) {
+            const int k0 = c * 32 + 8 * kq;  // A lane l: row l % 16, k = 8 (l / 16) + i
+            s8 av;
+            if constexpr (KC % 32 == 0) {
+                const int tap = k0 / KC, kc = k0 % KC;
+                const int pr = lr + 1 + (tap / 3 - 1), pc = lc + m + 1 + (tap % 3 - 1);
+                av = *reinterpret_cast<const s8*>(&Xs[(pr * PC + pc) * KC + kc]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int k = k0 + i;
+                    short v = 0;
+                    if (k < 9 * KC) {
+                        const int tap = k / KC, kc = k % KC;
+                        const int pr = lr + 1 + (tap / 3 - 1), pc = lc + m + 1 + (tap % 3 - 1);
+                        v = (short)Xs[(pr * PC + pc) * KC + kc];
+                    }
+                    av[i] = v;
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < NP / 16; ++nb) {
+                if constexpr (MODE == CONV_FWD_LAST)  // D[channel][pixel]
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, bv[c][nb]), __builtin_bit_cast(bf8, av), acc[nb], 0,
+                        0, 0);
+                else  // D[pixel][channel]
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv[c][nb]), acc[nb], 0,
+                        0, 0);
+            }
+        }
+        const size_t q0 = boff + ((size_t)(r0 + lr) << a.lg) + c0 + lc;  // pixel of block column 0
+        if constexpr (MODE == CONV_FWD_LAST) {
+            if (kq == 0) a.y[q0 + m] = (acc[0][0] + a.bias[0]) * a.yscale;
+        } else {
+#pragma unroll
+            for (int nb = 0; nb < NP / 16; ++nb) {
+                const int nn = nb * 16 + m;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const size_t q = q0 + 4 * kq + i;
+                    if (nn < NN) {
+                        const float zz = acc[nb][i] + a.bias[nn];
+                        if (a.z) a.z[q * NN + nn] = f2bf(zz);
+                        a.act[q * NN + nn] = f2bf(gelu(zz));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// forward layers: the tiled kernel where the grid is a multiple of its tile
+template <int KC, int NN, int MODE>
+void launch_conv_fwd(const ConvArgs& a, int batch, hipStream_t st) {
+    if (a.n % kTileCols == 0)
+        hipLaunchKernelGGL((k_conv_tile<KC, NN, MODE>),
+                           dim3((a.n / kTileCols) * (a.n / kTileRows), batch), dim3(512), 0, st, a);
+    else
+        launch_conv<KC, NN, MODE>(a, batch, st);
+}
+
 // dW[co][(tap, ci)] += sum_q dZ[q][co] X[q + off(tap)][ci] over this
 // workgroup's pixel range (all batch members flattened); dbias likewise.
 struct DwArgs {
@@ -406,9 +541,9 @@ int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStrea
         } else {
             a.y = y;
         }
-        if (l == 0) launch_conv<1, kHidden, CONV_FWD_GELU>(a, batch, st);
-        else if (l < 3) launch_conv<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
-        else launch_conv<kHidden, 1, CONV_FWD_LAST>(a, batch, st);
+        if (l == 0) launch_conv_fwd<1, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else if (l < 3) launch_conv_fwd<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else launch_conv_fwd<kHidden, 1, CONV_FWD_LAST>(a, batch, st);
     }
     return ck(c, hipGetLastError(), "closure forward") ? 3 : 0;
 }

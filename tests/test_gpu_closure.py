@@ -31,7 +31,8 @@ def make(ns, n, batch, seed=3):
     return L, h
 
 
-@pytest.mark.parametrize("n,batch", [(16, 1), (32, 3), (64, 2)])
+# n = 16, 32: per-tile kernel; n = 64, 128: the LDS-tiled forward kernel
+@pytest.mark.parametrize("n,batch", [(16, 1), (32, 3), (64, 2), (128, 2)])
 def test_forward_matches_oracle(ns, n, batch):
     L, h = make(ns, n, batch)
     rng = np.random.default_rng(n)
@@ -49,7 +50,7 @@ def test_forward_matches_oracle(ns, n, batch):
     L.nsd_closure_destroy(h)
 
 
-@pytest.mark.parametrize("n,batch", [(16, 2), (32, 2)])
+@pytest.mark.parametrize("n,batch", [(16, 2), (32, 2), (64, 1)])
 def test_gradient_matches_oracle(ns, n, batch):
     L, h = make(ns, n, batch, seed=11)
     rng = np.random.default_rng(100 + n)
