@@ -720,7 +720,9 @@ int nsd_closure_params(NsdClosure* c, float* params, int to_device) {
 
 int nsd_closure_forward(NsdClosure* c, const float* x, int batch, float* y) {
     if (!c || !x || !y || batch < 1 || batch > c->max_batch) return 4;
-    if (stage_input(c, x, batch) || launch_forward(c, batch)) return 3;
+    // inference: the pre-activations are not kept
+    if (stage_input(c, x, batch) || launch_forward_to(c, batch, c->y, 1.f, c->stream, false))
+        return 3;
     CKC(hipMemcpyAsync(y, c->y, sizeof(float) * batch * c->npix, hipMemcpyDeviceToHost, c->stream));
     CKC(hipStreamSynchronize(c->stream));
     return 0;
