@@ -61,6 +61,8 @@ int nsd_state_upload(NsdContext *c, const double *omega_hat);
 int nsd_state_download(NsdContext *c, double *omega_hat);
 int nsd_sfd_upload(NsdContext *c, const double *sfd_hat);
 int nsd_sfd_download(NsdContext *c, double *sfd_hat);
+/* random-phase initial vorticity scaled to `energy` (the same field for the
+ * same seed on every call) */
 int nsd_state_random_phase(NsdContext *c, unsigned long long seed, double k_peak, double energy);
 int nsd_set_time(NsdContext *c, double t, double dt);
 int nsd_get_time(NsdContext *c, double *t, double *dt, int *status);
@@ -75,7 +77,8 @@ int nsd_advance(NsdContext *c, int nsteps, int adaptive);
 int nsd_advance_graph(NsdContext *c, int nsteps, int adaptive);
 int nsd_state_derived(NsdContext *c, double *psi_hat, double *u_hat, double *v_hat, double *u,
                       double *v, double *omega);
-/* out = {E, Z} from the spectral state by Parseval, FP64 accumulation */
+/* out = {E, Z} from the spectral state by Parseval, FP64 accumulation in a
+ * fixed order (the same bits on every call for the same state) */
 int nsd_state_energy(NsdContext *c, double out[2]);
 int nsd_synchronize(NsdContext *c);
 
