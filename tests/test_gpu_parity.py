@@ -391,3 +391,25 @@ def test_forced_eager_steps_equal_graph_replay(ns, fp32):
     assert np.array_equal(w0_eager, w0_graph)
     assert t_eager == t_graph
     assert np.array_equal(w_eager, w_graph)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_state_energy_sums_are_reproducible(ns, fp32):
+    """nsd_state_energy sums workgroup partials in a fixed order: repeated
+    calls on one state return the same bits.  The random-phase field it
+    normalises has unit energy, and the Parseval energy agrees with the
+    reference's physical-space diagnostic (spectral_analysis.c:8-20)."""
+    s = solver(ns, 256, Re=1e4)
+    s.init(taylor_green=False)
+    s.set_precision(fp32)
+    s.random_phase(5, k_peak=10.0)
+    s.sync_device()
+    first = s.device_diagnostics()
+    for _ in range(3):
+        assert s.device_diagnostics() == first
+    e, z = first
+    tol = 1e-6 if fp32 else 1e-13
+    assert e == pytest.approx(1.0, rel=tol)
+    assert s.energy() == pytest.approx(e, rel=tol)
+    assert s.enstrophy() == pytest.approx(z, rel=tol)
+    s.close()
