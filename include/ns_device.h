@@ -118,8 +118,9 @@ int nsd_member_download(NsdContext *c, int member, double *omega_hat);
 /* Launch variants of the two time-step kernels (workgroup size and whether
  * inverse transforms run in lock-step pairs; lengths 64..1024 have several).
  * nsd_autotune times every variant on this context's shape (reps launches
- * each) and keeps the fastest; results are cached per (length, precision)
- * for the process and, if the environment names a file in NSD_TUNE_FILE,
+ * each) and keeps the fastest; results are cached per (length, precision,
+ * batch members) for the process and, if the environment names a file in
+ * NSD_TUNE_FILE (lines "row|col L precision members variant"),
  * across processes (force != 0 re-times).  It zeroes the state: call it
  * before setting the initial field.  Every variant gives identical results. */
 int nsd_autotune(NsdContext *c, int reps, int force, int *row_variant, int *col_variant);

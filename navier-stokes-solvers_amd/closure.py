@@ -54,6 +54,9 @@ def context(n, precision=NSD_FP32, members=1, phys=PHYSICS, seeds=None, force_se
         raise RuntimeError(err.value.decode())
     p = params(n, phys, force_seed)
     _check(L, L.nsd_set_params(h, C.byref(p)), h, "nsd_set_params")
+    # launch shapes timed on this batch (lengths 64..1024; the tuner clears
+    # the state, so it runs before the initial field), as ensembles do
+    _check(L, L.nsd_autotune(h, 3, 0, None, None), h, "nsd_autotune")
     if seeds is not None:
         sd = (C.c_ulonglong * members)(*[int(s) for s in seeds])
         _check(L, L.nsd_set_member_params(h, None, sd), h, "member params")

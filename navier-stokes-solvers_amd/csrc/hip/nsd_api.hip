@@ -1217,11 +1217,14 @@ int op_gather_prime(NsdContext* c, const double* in_ref) {
 
 // ----------------------------------------------------------------- autotuner
 
-// Tuned launch variants per (transform length, precision, kernel): timed once
-// per process, optionally persisted in the text file named by NSD_TUNE_FILE
-// (lines "row|col L precision variant").
+// Tuned launch variants per (kernel, transform length, precision, batch
+// members): the best launch shape depends on how many workgroups the batch
+// gives the device.  Timed once per process, optionally persisted in the text
+// file named by NSD_TUNE_FILE (lines "row|col L precision members variant";
+// lines of another form are skipped).
+using TuneKey = std::tuple<int, int, int, int>;
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, int> g_tune;
+std::map<TuneKey, int> g_tune;
 bool g_tune_loaded = false;
 
 void tune_load() {
@@ -1231,20 +1234,21 @@ void tune_load() {
     if (!f) return;
     FILE* fp = fopen(f, "r");
     if (!fp) return;
-    char kind[8];
-    int L, prec, v;
-    while (fscanf(fp, "%7s %d %d %d", kind, &L, &prec, &v) == 4)
-        g_tune[std::make_tuple(strcmp(kind, "col") == 0 ? 1 : 0, L, prec)] = v;
+    char line[128], kind[8];
+    int L, prec, nb, v;
+    while (fgets(line, sizeof(line), fp))
+        if (sscanf(line, "%7s %d %d %d %d", kind, &L, &prec, &nb, &v) == 5)
+            g_tune[TuneKey(strcmp(kind, "col") == 0 ? 1 : 0, L, prec, nb)] = v;
     fclose(fp);
 }
 
-void tune_store(int kind, int L, int prec, int v) {
-    g_tune[std::make_tuple(kind, L, prec)] = v;
+void tune_store(int kind, int L, int prec, int nb, int v) {
+    g_tune[TuneKey(kind, L, prec, nb)] = v;
     const char* f = getenv("NSD_TUNE_FILE");
     if (!f) return;
     FILE* fp = fopen(f, "a");
     if (!fp) return;
-    fprintf(fp, "%s %d %d %d\n", kind ? "col" : "row", L, prec, v);
+    fprintf(fp, "%s %d %d %d %d\n", kind ? "col" : "row", L, prec, nb, v);
     fclose(fp);
 }
 
@@ -1274,7 +1278,7 @@ int op_autotune(NsdContext* c, int reps, int force) {
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             tune_load();
-            auto it = g_tune.find(std::make_tuple(kind, L, prec));
+            auto it = g_tune.find(TuneKey(kind, L, prec, c->nb));
             if (it != g_tune.end() && !force) {
                 best = it->second;
                 cached = true;
@@ -1312,7 +1316,7 @@ int op_autotune(NsdContext* c, int reps, int force) {
                 }
             }
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            tune_store(kind, L, prec, best);
+            tune_store(kind, L, prec, c->nb, best);
         }
         if (kind == 0) c->vrow = best;
         else c->vcol = best;
