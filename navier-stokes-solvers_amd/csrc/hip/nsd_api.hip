@@ -16,6 +16,7 @@
 
 #include "ns_closure.h"
 #include "ns_device.h"
+#include "nsd_closure_link.hpp"
 #include "nsd_stage.hpp"
 
 using namespace nsd;
@@ -636,7 +637,6 @@ struct NsdContext {
     unsigned long long* m_ic = nullptr;     // per-member initial-field seeds, device
     NsdClosure* closure = nullptr;          // learned subgrid tendency (FP32, single field)
     void* cl_m = nullptr;                   // closure: mixed-layout scratch
-    float* cl_x = nullptr;                  // closure: physical vorticity
     float* cl_y = nullptr;                  // closure: physical tendency
     void* cl_src = nullptr;                 // closure: compact spectral tendency
     NsdParams P;
@@ -916,9 +916,10 @@ void enqueue_closure(NsdContext* c) {
         r.br = R_of(c);
         r.nfields = 1;
         r.mixed[0] = (const cx<T>*)c->cl_m;
-        r.out[0] = c->cl_x;
+        // the physical vorticity goes straight into the network's bf16 input
+        r.out_bf16 = closure_input(c->closure, &r.out_bf16_scale);
         launch_row_t<T>(c, r, c->G.ny);
-        (void)nsd_closure_eval_batch(c->closure, c->cl_x, c->cl_y, nb, (void*)c->stream);
+        (void)closure_eval_staged(c->closure, c->cl_y, nb, (void*)c->stream);
         RowArgs<T> r2 = row_args<T>(c);
         r2.mode = ROW_R2C;
         r2.nbat = nb;
@@ -1555,7 +1556,7 @@ void nsd_destroy(NsdContext* c) {
                     c->mixed[0], c->mixed[1], c->mixed[2], c->mixed[3], c->stH[0], c->stH[1],
                     c->stH[2], c->stH[3], c->stR[0], c->stR[1], c->stR[2], c->stR[3], c->fmodes,
                     c->fpartner, c->fou, c->fwork, c->fbox, c->m_inv_re, c->m_seed, c->m_ic,
-                    c->cl_m, c->cl_x, c->cl_y, c->cl_src};
+                    c->cl_m, c->cl_y, c->cl_src};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->fstream) (void)hipStreamSynchronize(c->fstream);
@@ -1773,7 +1774,6 @@ int nsd_attach_closure(NsdContext* c, NsdClosure* net) {
     }
     const size_t R = R_of(c) * c->nb;
     if (!c->cl_m) CK(hipMalloc(&c->cl_m, M_of(c) * c->esz * c->nb));
-    if (!c->cl_x) CK(hipMalloc(&c->cl_x, sizeof(float) * R));
     if (!c->cl_y) CK(hipMalloc(&c->cl_y, sizeof(float) * R));
     if (!c->cl_src) CK(hipMalloc(&c->cl_src, C_of(c) * c->esz * c->nb));
     c->closure = net;

@@ -26,6 +26,7 @@
 
 #include "ns_closure.h"
 #include "ns_device.h"
+#include "nsd_closure_link.hpp"
 
 namespace {
 
@@ -883,3 +884,17 @@ int nsd_closure_max_batch(const NsdClosure* c) { return c ? c->max_batch : 0; }
 int nsd_closure_size(const NsdClosure* c) { return c ? c->n : 0; }
 
 }  // extern "C"
+
+namespace nsd {
+
+unsigned short* closure_input(NsdClosure* c, float* in_scale) {
+    *in_scale = c->in_scale;
+    return c->x0;
+}
+
+int closure_eval_staged(NsdClosure* c, float* y, int batch, void* stream) {
+    if (!c || !y || batch < 1 || batch > c->max_batch) return 4;
+    return launch_forward_to(c, batch, y, c->out_scale, (hipStream_t)stream, false);
+}
+
+}  // namespace nsd

@@ -775,6 +775,8 @@ struct RowArgs {
     int nbat;              // members along grid y for the transform modes (0 = 1)
     int stagger_lo;        // ROW_STAGE: first-round workgroups [lo, hi) start late
     int stagger_hi;
+    unsigned short* out_bf16;  // ROW_C2R: field 0 as bf16, times out_bf16_scale (if non-null)
+    float out_bf16_scale;
 };
 
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
@@ -1042,6 +1044,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
                 for (int s = 0; s < P; ++s) {
                     const size_t o = (size_t)y * G.nx + j + s * TPL;
                     if (a.out_ref[f]) a.out_ref[f][o] = double(v[s].x);
+                    else if (f == 0 && a.out_bf16)  // round to nearest even
+                        a.out_bf16[roff + o] = __builtin_bit_cast(
+                            unsigned short, (__bf16)(a.out_bf16_scale * v[s].x));
                     else a.out[f][roff + o] = v[s].x;
                     if (B) {
                         if (a.out_ref[f + 1]) a.out_ref[f + 1][o] = double(v[s].y);
