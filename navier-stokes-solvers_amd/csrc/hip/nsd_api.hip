@@ -1580,6 +1580,10 @@ int nsd_set_params(NsdContext* c, const NsdParams* p) {
         c->err = "selective frequency damping requires the classic scheme";
         return ST_PARAM;
     }
+    if (!(p->drag >= 0.0) || (p->hv_on && !(p->hv_coeff >= 0.0))) {
+        c->err = "drag and the hyperviscosity coefficient must not be negative";
+        return ST_PARAM;
+    }
     c->P = *p;
     Phys& f = c->phys;
     f.inv_Re = 1.0 / p->Re;

@@ -413,3 +413,22 @@ def test_state_energy_sums_are_reproducible(ns, fp32):
     assert s.energy() == pytest.approx(e, rel=tol)
     assert s.enstrophy() == pytest.approx(z, rel=tol)
     s.close()
+
+
+def test_device_params_reject_negative_drag(ns):
+    """nsd_set_params rejects what ns_spectral_set_drag rejects (mu < 0),
+    with a message, and accepts the corrected parameters."""
+    import ctypes as C
+    import importlib
+    cl = importlib.import_module(ns.__name__ + ".closure")
+    L = ns.load()
+    err = C.create_string_buffer(256)
+    h = L.nsd_create_batch(64, 64, cl.TWO_PI, cl.TWO_PI, ns.NSD_FP32, 1, err, 256)
+    assert h, err.value
+    p = cl.params(64)
+    p.drag = -0.1
+    assert L.nsd_set_params(h, C.byref(p)) == 4
+    assert b"drag" in L.nsd_error(h)
+    p.drag = 0.1
+    assert L.nsd_set_params(h, C.byref(p)) == 0
+    L.nsd_destroy(h)
