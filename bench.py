@@ -228,16 +228,17 @@ def cpu_baseline(n):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--gpus", type=int, default=1, help="GPUs (this build measures one)")
+    ap.add_argument("--steps", type=int, default=50, help="timed steps (K)")
     # about 70 ms of load at 4096^2: after 10 warm-up steps the timed steps
     # still run ~2 % slow while the clocks ramp up
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--n", type=int, default=4096, help="grid size N of the N^2 workload")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline (one oracle step, about 20 s)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 step")
     ap.add_argument("--no-closure", action="store_true",
-                    help="skip training and evaluating the learned closure (about 30 s)")
+                    help="skip training and evaluating the learned closure (about 20 s)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-member mixed-size ensemble")
     ap.add_argument("--no-compat", action="store_true",
