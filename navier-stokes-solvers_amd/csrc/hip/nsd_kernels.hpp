@@ -169,7 +169,7 @@ __device__ __forceinline__ T expm1_t(T x) {
         const float e = __expf(x) - 1.f;
         return fabsf(x) < 0.35f ? p : e;
     } else {
-        // e^x < 2^-54 for x < -40, so expm1 rounds to -1 there: the modes that
+        // e^x < 2^-54 for x < -40, so -1 is expm1 correctly rounded: the modes that
         // the hyperviscous factor damps completely (most of the high
         // wavenumbers) skip the library call
         if (x < -40.0) return -1.0;
