@@ -163,6 +163,7 @@ __device__ __forceinline__ T sfd_weight_t(const Phys& p, T k2) {
 template <typename T>
 __device__ __forceinline__ T expm1_t(T x) {
     if constexpr (sizeof(T) == 4) {
+        if (x < -18.f) return -1.f;  // e^x < 2^-25: -1 is expm1 correctly rounded
         const float p =
             x * (1.f + x * (0.5f + x * (1.f / 6 + x * (1.f / 24 + x * (1.f / 120 +
                  x * (1.f / 720 + x * (1.f / 5040)))))));
