@@ -718,6 +718,14 @@ int lastk(NsdContext* c) {
     return ST_OK;
 }
 
+// The stage kernels' mixed fields use the pair layout (pair_layout in
+// nsd_kernels.hpp) on square FP32 4096^2 grids with the default launch
+// shapes, the only stage kernels compiled with it.
+bool pairs_ok(const NsdContext* c) {
+    return c->prec == NSD_FP32 && c->G.nx == 4096 && c->G.ny == 4096 && c->vrow == 0 &&
+           c->vcol == 0;
+}
+
 template <typename T>
 ColArgs<T> col_args(NsdContext* c) {
     ColArgs<T> a;
@@ -739,6 +747,7 @@ ColArgs<T> col_args(NsdContext* c) {
     a.bf = (size_t)c->fbx * (2 * c->fby + 1);
     a.m_inv_re = c->m_inv_re;
     a.src = c->closure ? (const cx<T>*)c->cl_src : nullptr;
+    a.pairs = pairs_ok(c) ? 1 : 0;
     return a;
 }
 
@@ -752,6 +761,7 @@ RowArgs<T> row_args(NsdContext* c) {
     a.nl = (cx<T>*)c->nl;
     a.sc = c->sc;
     a.bm = M_of(c);
+    a.pairs = pairs_ok(c) ? 1 : 0;
     return a;
 }
 
@@ -1129,6 +1139,7 @@ int op_nonlinear(NsdContext* c, const double* w_ref, const double* psi_ref, doub
     RowArgs<T> r = row_args<T>(c);
     r.mode = ROW_STAGE;
     r.track_umax = 0;
+    r.pairs = 0;  // the transforms above wrote the per-field layout
     launch_row_t<T>(c, r, c->G.ny / 2);
     ColArgs<T> f = col_args<T>(c);
     f.mode = COL_FWD;
@@ -1504,8 +1515,12 @@ NsdContext* nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision
         bad(hipMalloc(&c->stagebuf, C)) || bad(hipMalloc(&c->sfd, C)) ||
         bad(hipMalloc(&c->nl, M)) || bad(hipMalloc(&c->sc, S)))
         return nullptr;
-    for (int i = 0; i < 4; ++i)
-        if (bad(hipMalloc(&c->mixed[i], M))) return nullptr;
+    // psi_x, psi_y in one allocation and omega_x, omega_y in another, so that
+    // each pair can also be held interleaved (pair layout)
+    for (int i = 0; i < 4; i += 2) {
+        if (bad(hipMalloc(&c->mixed[i], 2 * M))) return nullptr;
+        c->mixed[i + 1] = (char*)c->mixed[i] + M;
+    }
     if (bad(hipMemset(c->state, 0, C)) || bad(hipMemset(c->sfd, 0, C)) ||
         bad(hipMemset(c->sc, 0, S)))
         return nullptr;
@@ -1553,7 +1568,7 @@ void nsd_destroy(NsdContext* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto e : c->evs) (void)hipEventDestroy(e);
     void* bufs[] = {c->state, c->acc, c->stagebuf, c->sfd, c->nl, c->twx, c->twy, c->sc,
-                    c->mixed[0], c->mixed[1], c->mixed[2], c->mixed[3], c->stH[0], c->stH[1],
+                    c->mixed[0], c->mixed[2], c->stH[0], c->stH[1],
                     c->stH[2], c->stH[3], c->stR[0], c->stR[1], c->stR[2], c->stR[3], c->fmodes,
                     c->fpartner, c->fou, c->fwork, c->fbox, c->m_inv_re, c->m_seed, c->m_ic,
                     c->cl_m, c->cl_y, c->cl_src};

@@ -368,6 +368,7 @@ struct ColArgs {
     const cx<T>* tw;  // twiddles for length ny
     const cx<T>* nl;  // mixed input (COL_STAGE, COL_FWD)
     cx<T>* mixed[4];  // mixed outputs
+    int pairs;        // COL_STAGE, COL_PRIME: write them in the pair layout (pair_layout)
     cx<T>* state;     // compact state (read; written on the last stage)
     cx<T>* acc;       // compact RK accumulator
     cx<T>* stagebuf;  // compact stage state (classic scheme)
@@ -411,6 +412,28 @@ __device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
 }
 
 // occupancy floor handed to the register allocator: waves per SIMD
+// Pair layout of the stage kernels' mixed fields (FP32, 4096-point lines):
+// (psi_x, psi_y) interleaved per (kx, y) in the buffer of psi_x and psi_y,
+// (omega_x, omega_y) likewise in that of omega_x and omega_y; pair element e
+// = kx ny + y of member m sits at 2 (m M + e) of the pair's buffer (M = kept
+// kx times ny).  A row pair's two 16-byte gathers of one pair then fall in
+// one 64-byte line, where the per-field layout puts a field's row pair in a
+// line of its own: half the L1 misses of the row kernel's gathers, whose
+// address path otherwise waits on the cache for about half of its cycles.
+// k_col still writes whole lines (one 16-byte store per lane and pair).  The
+// host selects it per launch (ColArgs/RowArgs::pairs) for square FP32 4096^2
+// grids; the transform modes keep the per-field layout.
+template <typename T, int LC>
+constexpr bool pair_layout() {
+    return sizeof(T) == 4 && LC == 4096;
+}
+
+// store (first, second) at pair element e of pair buffer B (one 16 bytes)
+__device__ __forceinline__ void store_pair(cx<float>* B, size_t e, cx<float> first,
+                                           cx<float> second) {
+    *reinterpret_cast<float4*>(B + 2 * e) = make_float4(first.x, first.y, second.x, second.y);
+}
+
 template <typename T>
 constexpr int min_waves() { return sizeof(T) == 4 ? 4 : 2; }
 
@@ -670,12 +693,21 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             TwRegs<LC, T> tw;
             load_tw_regs<LC, 0>(tw, j, a.tw);
             if (has_data) {
-                cx<T>* o1 = a.mixed[3] + mixed_off;
-                cx<T>* o2 = a.mixed[2] + mixed_off;
+                if constexpr (pair_layout<T, LC>()) {
+                    if (a.pairs) {  // (omega_x, omega_y)
+                        cx<T>* pb = a.mixed[2] + 2 * mixed_off;
 #pragma unroll
-                for (int s = 0; s < P; ++s) {
-                    o1[mbase + j + s * TPL] = v[s];
-                    o2[mbase + j + s * TPL] = v2[s];
+                        for (int s = 0; s < P; ++s) store_pair(pb, mbase + j + s * TPL, v2[s], v[s]);
+                    }
+                }
+                if (!(pair_layout<T, LC>() && a.pairs)) {
+                    cx<T>* o1 = a.mixed[3] + mixed_off;
+                    cx<T>* o2 = a.mixed[2] + mixed_off;
+#pragma unroll
+                    for (int s = 0; s < P; ++s) {
+                        o1[mbase + j + s * TPL] = v[s];
+                        o2[mbase + j + s * TPL] = v2[s];
+                    }
                 }
             }
 #pragma unroll
@@ -688,12 +720,21 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
             line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_y, psi_x
             if (has_data) {
-                cx<T>* o1 = a.mixed[1] + mixed_off;
-                cx<T>* o2 = a.mixed[0] + mixed_off;
+                if constexpr (pair_layout<T, LC>()) {
+                    if (a.pairs) {  // (psi_x, psi_y)
+                        cx<T>* pb = a.mixed[0] + 2 * mixed_off;
 #pragma unroll
-                for (int s = 0; s < P; ++s) {
-                    o1[mbase + j + s * TPL] = v[s];
-                    o2[mbase + j + s * TPL] = v2[s];
+                        for (int s = 0; s < P; ++s) store_pair(pb, mbase + j + s * TPL, v2[s], v[s]);
+                    }
+                }
+                if (!(pair_layout<T, LC>() && a.pairs)) {
+                    cx<T>* o1 = a.mixed[1] + mixed_off;
+                    cx<T>* o2 = a.mixed[0] + mixed_off;
+#pragma unroll
+                    for (int s = 0; s < P; ++s) {
+                        o1[mbase + j + s * TPL] = v[s];
+                        o2[mbase + j + s * TPL] = v2[s];
+                    }
                 }
             }
             return;
@@ -784,6 +825,7 @@ struct RowArgs {
     int stagger_hi;
     unsigned short* out_bf16;  // ROW_C2R: field 0 as bf16, times out_bf16_scale (if non-null)
     float out_bf16_scale;
+    int pairs;                 // ROW_STAGE: mixed fields in the pair layout (pair_layout)
 };
 
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
@@ -965,6 +1007,32 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
     }
 }
 
+// Both fields of a pair-layout buffer at rows y, y + 1 (f0: first, f1:
+// second): two 16-byte loads per slot from one 64-byte line.
+template <int LC>
+__device__ __forceinline__ void gather_pair_rows(RowPairRegs<float>& f0, RowPairRegs<float>& f1,
+                                                 int j, const Geo& G, const cx<float>* B, int y,
+                                                 bool active) {
+    constexpr int kxm = Band<LC>::kxm;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+        const int kind = slot_kind(s, LC, kxm);
+        const int k = j + s * (LC / P);
+        if (kind == 1 || (kind == 3 && k <= kxm)) {
+            if (active) {
+                const float4* src = reinterpret_cast<const float4*>(at32(B, 2 * col_elem(k, G.ny, y)));
+                const float4 r0 = src[0], r1 = src[1];  // rows y, y + 1
+                f0.a[s] = mk(r0.x, r0.y);
+                f1.a[s] = mk(r0.z, r0.w);
+                f0.b[s] = mk(r1.x, r1.y);
+                f1.b[s] = mk(r1.z, r1.w);
+            } else {
+                f0.a[s] = f1.a[s] = f0.b[s] = f1.b[s] = mk(0.f, 0.f);
+            }
+        }
+    }
+}
+
 // The LDS image of stage_two_rows, from gathered registers.
 template <int LC, typename T>
 __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const RowPairRegs<T>& r) {
@@ -1097,12 +1165,24 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         // the compiler's schedule of the loads above within 128 VGPRs)
         if (blk * lines < G.ny / 2) {
             RowPairRegs<T> r0, r1, q0, q1;
-            gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
-            gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
+            bool pl = false;
+            if constexpr (pair_layout<T, LC>()) pl = a.pairs != 0;
+            if constexpr (pair_layout<T, LC>()) {
+                if (pl) gather_pair_rows<LC>(r0, r1, j, G, a.mixed[0] + 2 * moff, y0, active);
+            }
+            if (!pl) {
+                gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
+                gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
+            }
             stage_two_rows_regs<LC>(lds, j, r0);
             stage_two_rows_regs<LC>(lds2, j, r1);
-            gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
-            gather_two_rows<LC>(q1, j, G, a.mixed[3] + moff, y0, active);
+            if constexpr (pair_layout<T, LC>()) {
+                if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
+            }
+            if (!pl) {
+                gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
+                gather_two_rows<LC>(q1, j, G, a.mixed[3] + moff, y0, active);
+            }
             __syncthreads();
             cx<T> v2[P];
             lds_read_c<LC>(v, lds, j);
