@@ -137,3 +137,35 @@ def test_config3_forced_4096_fp32_vs_fp64(ns):
     assert worst_e <= 1e-5 and worst_z <= 1e-5
     for s in runs:
         s.close()
+
+
+def test_pair_layout_transitions_4096_fp32(ns):
+    """At FP32 4096^2 the stage kernels hold the mixed fields in the pair
+    layout (psi and omega derivatives interleaved per (kx, y)); the compat
+    transforms use the per-field one.  The nonlinear-term entry point, whose
+    per-field transforms feed the row stage, agrees with its FP64 result to
+    FP32 round-off; and a step taken after it (the prime pass restores the
+    pair layout) equals, bit for bit, a step taken without it."""
+    n = 4096
+    made = []
+    for fp32, compat in ((False, True), (True, True), (True, False)):
+        s = ns.Spectral(n, n, TWO_PI, TWO_PI, 1e4)
+        s.init(taylor_green=False)
+        s.set_precision(fp32)
+        s.random_phase(3, k_peak=10.0)
+        if compat:
+            s.L.ns_spectral_compute_velocity_from_vorticity(s.p)
+            s.L.ns_spectral_compute_nonlinear_terms(s.p)
+            s.check()
+        made.append(s)
+    f64, f32, plain = made
+    err = rel_l2(f32.spec("nonlinear_hat"), f64.spec("nonlinear_hat"))
+    print(f"nonlinear term FP32 vs FP64 at 4096^2: rel L2 {err:.2e}", flush=True)
+    assert err < 1e-5
+    for s in (f32, plain):
+        assert s.advance_n(1, True) == 0
+        assert s.sync_host() == 0
+    assert f32.d.t == plain.d.t
+    assert np.array_equal(f32.spec("omega_hat"), plain.spec("omega_hat"))
+    for s in made:
+        s.close()
