@@ -1893,9 +1893,13 @@ int nsd_set_variants(NsdContext* c, int row_variant, int col_variant) {
         c->err = "launch variant not available for this size";
         return ST_PARAM;
     }
+    const bool pairs_before = pairs_ok(c);
     c->vrow = row_variant;
     c->vcol = col_variant;
     c->gen++;
+    // the mixed fields' layout follows the variants: rewrite them from the
+    // state when it changes
+    if (pairs_ok(c) != pairs_before) return TYPED(op_prime, c);
     return ST_OK;
 }
 

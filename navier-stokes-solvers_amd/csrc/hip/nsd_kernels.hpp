@@ -422,7 +422,9 @@ __device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
 // address path otherwise waits on the cache for about half of its cycles.
 // k_col still writes whole lines (one 16-byte store per lane and pair).  The
 // host selects it per launch (ColArgs/RowArgs::pairs) for square FP32 4096^2
-// grids; the transform modes keep the per-field layout.
+// grids with the default launch shapes; the transform modes keep the
+// per-field layout.  At 64..2048 it measured slower (256-member ensemble
+// 92k -> 86k member-steps/s), so it stays at 4096.
 template <typename T, int LC>
 constexpr bool pair_layout() {
     return sizeof(T) == 4 && LC == 4096;
