@@ -199,6 +199,10 @@ int ns_spectral_advance_n(NSSpectralData *data, int nsteps, int adaptive);
 int ns_spectral_sync_host(NSSpectralData *data);
 /* energy and enstrophy of the device state (Parseval, FP64 sums) */
 int ns_spectral_device_diagnostics(NSSpectralData *data, double *energy, double *enstrophy);
+/* the numbers ns_spectral_analyze_convergence prints (spectral_analysis.c:69-100),
+ * from the host arrays: out = {E, Z, max|omega_hat|, spectral error (RMS of
+ * |omega_hat| over the modes with k^2 > k2[last] / 4), t, dt} */
+int ns_spectral_convergence_metrics(NSSpectralData *data, double out[6]);
 
 /* apply ns_config keys (fluid.Re, time.dt, spectral.*, forcing.*; see the
  * key table in ns_config.c) to an allocated instance */

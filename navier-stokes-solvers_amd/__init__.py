@@ -45,7 +45,7 @@ EXPORTED = [
     "ns_spectral_set_scheme", "ns_spectral_set_drag", "ns_spectral_set_forcing",
     "ns_spectral_initialize_random_phase", "ns_spectral_sync_device", "ns_spectral_advance_n",
     "ns_spectral_sync_host", "ns_spectral_device_diagnostics", "ns_spectral_configure",
-    "ns_spectral_create_from_config",
+    "ns_spectral_create_from_config", "ns_spectral_convergence_metrics",
     # forcing.h
     "ns_forcing_config_default", "ns_forcing_state_init", "ns_forcing_state_free",
     "ns_forcing_generate",
@@ -211,6 +211,7 @@ def load():
                      ("ns_spectral_sync_device", []), ("ns_spectral_sync_host", []),
                      ("ns_spectral_advance_n", [C.c_int, C.c_int]),
                      ("ns_spectral_device_diagnostics", [c_double_p, c_double_p]),
+                     ("ns_spectral_convergence_metrics", [c_double_p]),
                      ("ns_spectral_configure", [C.POINTER(NSConfig)])):
         getattr(L, n).argtypes = [P] + extra
         getattr(L, n).restype = C.c_int
@@ -530,6 +531,12 @@ class Spectral:
         e, z = C.c_double(), C.c_double()
         self.check(self.L.ns_spectral_device_diagnostics(self.p, C.byref(e), C.byref(z)))
         return e.value, z.value
+
+    def convergence_metrics(self):
+        """{E, Z, max|omega_hat|, spectral error, t, dt} of analyze_convergence"""
+        out = np.zeros(6)
+        self.check(self.L.ns_spectral_convergence_metrics(self.p, _dp(out)))
+        return out
 
     @property
     def ctx(self):

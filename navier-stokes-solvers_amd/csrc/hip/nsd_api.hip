@@ -675,6 +675,9 @@ struct NsdContext {
     std::string err;
 };
 
+// profiling events per step: three around the two kernels of each stage
+constexpr int kEvPerStep = 12;
+
 #define CK(call)                                                                   \
     do {                                                                           \
         hipError_t e_ = (call);                                                    \
@@ -719,11 +722,12 @@ int lastk(NsdContext* c) {
 }
 
 // The stage kernels' mixed fields use the pair layout (pair_layout in
-// nsd_kernels.hpp) on square FP32 4096^2 grids with the default launch
-// shapes, the only stage kernels compiled with it.
+// nsd_kernels.hpp) on square FP32 4096^2 grids.  Length 4096 has only the
+// default launch variant (var_ok in nsd_stage.hip), and the kernels
+// static_assert that every instantiation pair_layout selects holds the pair
+// paths, so the layout cannot change with nsd_set_variants.
 bool pairs_ok(const NsdContext* c) {
-    return c->prec == NSD_FP32 && c->G.nx == 4096 && c->G.ny == 4096 && c->vrow == 0 &&
-           c->vcol == 0;
+    return c->prec == NSD_FP32 && c->G.nx == 4096 && c->G.ny == 4096;
 }
 
 template <typename T>
@@ -866,11 +870,16 @@ void enqueue_stage(NsdContext* c, int s, int adaptive) {
     a.mode = COL_STAGE;
     a.stage = s;
     a.phys.adaptive = adaptive;
+    // profiling: three events per stage bracket exactly the two kernels
+    // (before k_row, between the kernels, after k_col), so neither bracket
+    // holds the forcing stream's join or another stage's event
     const bool prof = c->prof != 0;
-    if (prof) (void)hipEventRecord(c->evs[c->prof_steps * 8 + 2 * s], c->stream);
+    const size_t e0 = (size_t)c->prof_steps * kEvPerStep + 3 * s;
+    if (prof) (void)hipEventRecord(c->evs[e0], c->stream);
     launch_row_t<T>(c, r, c->G.ny / 2);
-    if (prof) (void)hipEventRecord(c->evs[c->prof_steps * 8 + 2 * s + 1], c->stream);
+    if (prof) (void)hipEventRecord(c->evs[e0 + 1], c->stream);
     launch_col_t<T>(c, a, c->G.nkx_keep);
+    if (prof) (void)hipEventRecord(c->evs[e0 + 2], c->stream);
 }
 
 template <typename T>
@@ -970,38 +979,37 @@ void enqueue_step(NsdContext* c, int adaptive) {
 
 template <typename T>
 int op_advance(NsdContext* c, int nsteps, int adaptive) {
+    const size_t last = (size_t)nsteps * kEvPerStep;
     if (c->prof) {
-        const size_t need = (size_t)nsteps * 8 + 2;
+        const size_t need = last + 2;
         while (c->evs.size() < need) {
             hipEvent_t e;
             CK(hipEventCreate(&e));
             c->evs.push_back(e);
         }
         c->prof_steps = 0;
-        CK(hipEventRecord(c->evs[(size_t)nsteps * 8], c->stream));
+        CK(hipEventRecord(c->evs[last], c->stream));
     }
     for (int n = 0; n < nsteps; ++n) {
         enqueue_step<T>(c, adaptive);
         if (c->prof) c->prof_steps++;
     }
     if (c->prof) {
-        CK(hipEventRecord(c->evs[(size_t)nsteps * 8 + 1], c->stream));
+        CK(hipEventRecord(c->evs[last + 1], c->stream));
         CK(hipStreamSynchronize(c->stream));
         double row = 0, col = 0;
         for (int n = 0; n < nsteps; ++n) {
             for (int s = 0; s < 4; ++s) {
+                const size_t e0 = (size_t)n * kEvPerStep + 3 * s;
                 float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, c->evs[n * 8 + 2 * s], c->evs[n * 8 + 2 * s + 1]);
+                (void)hipEventElapsedTime(&ms, c->evs[e0], c->evs[e0 + 1]);
                 row += ms;
-                hipEvent_t nxt = (s < 3) ? c->evs[n * 8 + 2 * s + 2]
-                                         : (n + 1 < nsteps ? c->evs[(n + 1) * 8]
-                                                           : c->evs[(size_t)nsteps * 8 + 1]);
-                (void)hipEventElapsedTime(&ms, c->evs[n * 8 + 2 * s + 1], nxt);
+                (void)hipEventElapsedTime(&ms, c->evs[e0 + 1], c->evs[e0 + 2]);
                 col += ms;
             }
         }
         float tot = 0.f;
-        (void)hipEventElapsedTime(&tot, c->evs[(size_t)nsteps * 8], c->evs[(size_t)nsteps * 8 + 1]);
+        (void)hipEventElapsedTime(&tot, c->evs[last], c->evs[last + 1]);
         c->prof_row = row / (4.0 * nsteps);
         c->prof_col = col / (4.0 * nsteps);
         c->prof_step = tot / nsteps;
@@ -1848,6 +1856,11 @@ int nsd_subgrid_sample(NsdContext* fine, int member, NsdContext* coarse, float* 
     hipLaunchKernelGGL(k_d2f, dim3(grid_for(Rc)), dim3(256), 0, c->stream, Rc, c->stR[0], x);
     hipLaunchKernelGGL(k_d2f, dim3(grid_for(Rc)), dim3(256), 0, c->stream, Rc, c->stR[1], t);
     if (lastk(c)) return ST_MEM;
+    // the inverse transforms above reused the coarse context's mixed buffers:
+    // restore the invariant that they hold the derivative fields of its state
+    // (the filtered fine field), so that a step taken on it next starts right
+    rc = TYPED(restore_mixed, c);
+    if (rc) return rc;
     return sync(c);
 }
 
@@ -1893,13 +1906,9 @@ int nsd_set_variants(NsdContext* c, int row_variant, int col_variant) {
         c->err = "launch variant not available for this size";
         return ST_PARAM;
     }
-    const bool pairs_before = pairs_ok(c);
     c->vrow = row_variant;
     c->vcol = col_variant;
     c->gen++;
-    // the mixed fields' layout follows the variants: rewrite them from the
-    // state when it changes
-    if (pairs_ok(c) != pairs_before) return TYPED(op_prime, c);
     return ST_OK;
 }
 

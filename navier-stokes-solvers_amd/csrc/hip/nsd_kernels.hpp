@@ -452,6 +452,11 @@ struct TwSource<true, L, T> {
 
 template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
+    // a pair-layout length must instantiate the paired writer (the host
+    // selects the layout from the grid alone)
+    static_assert(!(MODE == COL_STAGE || MODE == COL_PRIME) || !pair_layout<T, LC>() ||
+                      paired_lines<T, LC, NT, PR>(),
+                  "pair layout needs the paired k_col path");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     const Geo& G = a.geo;
@@ -1076,6 +1081,9 @@ constexpr bool row_prefetch() {
 
 template <typename T, int NT, int MODE, int LC, int PR = -1>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
+    static_assert(MODE != ROW_STAGE || !pair_layout<T, LC>() ||
+                      (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()),
+                  "pair layout needs the paired gather-ahead k_row path");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     __shared__ double red[NT / 64];

@@ -266,16 +266,35 @@ int ns_spectral_check_resolution(NSSpectralData *d) {
     return 1;
 }
 
+/* spectral_analysis.c:71-96: E and Z of the physical arrays, then over the
+ * stored modes max|omega_hat| and the RMS of |omega_hat| where
+ * k2 > 0.25 k2[last] (k2[last] is the last stored mode, quirk Q7) */
+int ns_spectral_convergence_metrics(NSSpectralData *d, double out[6]) {
+    if (!ready(d) || !out) return SOLVER_INVALID_PARAM;
+    const double energy = ns_spectral_compute_energy(d);
+    if (ns_spectral_last_status(d) != SOLVER_SUCCESS) return ns_spectral_last_status(d);
+    const double enstrophy = ns_spectral_compute_enstrophy(d);
+    if (ns_spectral_last_status(d) != SOLVER_SUCCESS) return ns_spectral_last_status(d);
+    double s[5] = {0, 0, 0, 0, 0};
+    int rc = nsd_spec_reduce(d->device->ctx, (const double *)d->omega_hat, s);
+    if (rc) return set_status(d, rc);
+    out[0] = energy;
+    out[1] = enstrophy;
+    out[2] = s[2];
+    out[3] = (s[4] > 0.0) ? sqrt(s[3] / s[4]) : 0.0; /* spectral_analysis.c:96 */
+    out[4] = d->t;
+    out[5] = d->dt;
+    return set_status(d, 0);
+}
+
 void ns_spectral_analyze_convergence(NSSpectralData *d, int step) {
     if (!ready(d) || step % 10 != 0) return;
-    const double energy = ns_spectral_compute_energy(d);
-    const double enstrophy = ns_spectral_compute_enstrophy(d);
-    double s[5] = {0, 0, 0, 0, 0};
-    if (set_status(d, nsd_spec_reduce(d->device->ctx, (const double *)d->omega_hat, s))) return;
-    const double spectral_error = (s[4] > 0.0) ? sqrt(s[3] / s[4]) : 0.0;
+    double m[6];
+    if (ns_spectral_convergence_metrics(d, m)) return;
     /* spectral_analysis.c:98-99 */
     printf("Step %d: t=%.4f, E=%.6e, \xce\xa9=%.6e, max|\xcf\x89\xcc\x82|=%.2e, err=%.2e, dt=%.2e\n",
-           step, d->t, energy, enstrophy, s[2], spectral_error, d->dt);
+           step, m[4], m[0], m[1], m[2], m[3], m[5]);
+    fflush(stdout);
 }
 
 void ns_spectral_analyze_spectrum(NSSpectralData *d) {

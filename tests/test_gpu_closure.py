@@ -225,3 +225,29 @@ def test_trained_closure_beats_the_plain_coarse_solve(ns):
     assert r["closure"]["spectrum_mean_rel_err"] < 0.5 * r["plain"]["spectrum_mean_rel_err"]
     assert r["closure_accepted"] and r["result"] == "closure"
     net.close()
+
+
+def test_rejected_closure_falls_back_to_the_fine_solve(ns):
+    """A network that drives the enstrophy away from the fine solve (its only
+    non-zero parameter is a large output bias, which raises the mean
+    vorticity at 400 per unit time) is rejected by the 10 % enstrophy-drift
+    rule, and the result is the fine solve itself (filtered to the coarse
+    grid), not the closure run and not the plain coarse solve."""
+    import importlib
+    cl = importlib.import_module(ns.__name__ + ".closure")
+    net = cl.Net(64, max_batch=4, seed=1)
+    P = net.L.nsd_closure_param_count(net.h)
+    p = np.zeros(P, np.float32)
+    p[-1] = 1.0
+    assert net.L.nsd_closure_params(net.h, fp(p), 1) == 0
+    assert net.L.nsd_closure_set_scales(net.h, 1.0, 400.0) == 0
+    r = cl.evaluate(net, 256, [5, 6, 7, 8], 0.05)
+    print(f"closure enstrophy drift {r['closure']['enstrophy_rel_err']:.3f} -> {r['result']}",
+          flush=True)
+    assert r["closure"]["enstrophy_rel_err"] > 0.10
+    assert not r["closure_accepted"] and r["result"] == "fine (fallback)"
+    assert r["result_speedup_vs_fine"] == 1.0
+    assert len(r["fields"]) == 4
+    for got, fine in zip(r["fields"], r["fine_filtered"]):
+        assert got.shape == (64, 33) and np.array_equal(got, fine)
+    net.close()

@@ -14,7 +14,9 @@ by orders of magnitude.
 4096^2 forced turbulence (the throughput workload) in FP32 stages against
 the same run in FP64 on the GPU: single-step agreement to FP32 round-off
 (1e-5 relative L2), then the error growth of energy, enstrophy and the field
-over 2000 steps, printed, with energy and enstrophy held to 1e-5 relative.
+over 2000 steps, printed every 200 steps, with energy and enstrophy held to
+1e-5 relative and the field to 1e-4 (its chaotic growth envelope).  The FP64
+run itself is checked against the oracle at this size in test_gpu_headline.
 The integrating factors are carried as E - 1 so that the slow decay of the
 largest scales (L dt ~ 1e-5) is not rounded away in FP32; with E rounded to
 FP32 directly, energy drifted by 6e-5 over these 2000 steps.
@@ -107,6 +109,7 @@ def forced_4096(ns, fp32):
 
 
 def test_config3_forced_4096_fp32_vs_fp64(ns):
+    n_grid = 4096
     runs = [forced_4096(ns, False), forced_4096(ns, True)]
     # one step: FP32 round-off only
     for s in runs:
@@ -117,23 +120,31 @@ def test_config3_forced_4096_fp32_vs_fp64(ns):
     print(f"1 step: omega_hat rel L2 {e1:.2e}", flush=True)
     assert e1 <= 1e-5
     worst_e = worst_z = 0.0
+    curve = []
     for chunk in range(1, 11):
-        diag = []
+        diag, states = [], []
         for s in runs:
             assert s.advance_n(200 if chunk > 1 else 199, True) == 0
             diag.append(s.device_diagnostics())
+            w = np.zeros((n_grid, n_grid // 2 + 1), np.complex128)
+            s.check(s.L.nsd_state_download(s.ctx, ns._dp(w)))
+            states.append(w)
         (E64, Z64), (E32, Z32) = diag
         de, dz = abs(E32 / E64 - 1), abs(Z32 / Z64 - 1)
+        ef = rel_l2(states[1], states[0])
+        curve.append(ef)
         worst_e, worst_z = max(worst_e, de), max(worst_z, dz)
         print(f"step {chunk * 200}: t64 {runs[0].d.t:.6f} t32 {runs[1].d.t:.6f} "
-              f"E {E64:.6e} rel err {de:.1e}  Z {Z64:.6e} rel err {dz:.1e}", flush=True)
-    for s in runs:
-        assert s.sync_host() == 0
-    w64, w32 = (s.spec("omega_hat") for s in runs)
-    ef = rel_l2(w32, w64)
-    print(f"2000 steps: omega_hat rel L2 {ef:.2e}; worst E {worst_e:.1e}, Z {worst_z:.1e}",
-          flush=True)
-    assert np.isfinite(ef)
+              f"E {E64:.6e} rel err {de:.1e}  Z {Z64:.6e} rel err {dz:.1e}  "
+              f"omega_hat rel L2 {ef:.2e}", flush=True)
+    print("growth curve (every 200 steps): " + " ".join(f"{e:.2e}" for e in curve), flush=True)
+    print(f"2000 steps: omega_hat rel L2 {curve[-1]:.2e}; worst E {worst_e:.1e}, "
+          f"Z {worst_z:.1e}", flush=True)
+    # Pointwise agreement of a chaotic flow grows from the one-step FP32
+    # round-off (~1e-7) at the flow's Lyapunov rate; over these 2000 steps
+    # (t ~ 0.37) the growth stays within two decades, so 1e-4 bounds it while
+    # any difference of algorithm (a term, a sign, a mode) is O(1e-2) or more.
+    assert all(np.isfinite(curve)) and max(curve) <= 1e-4
     assert worst_e <= 1e-5 and worst_z <= 1e-5
     for s in runs:
         s.close()

@@ -1,0 +1,123 @@
+"""The throughput workload at its own sizes against the CPU oracle.
+
+Forced 2-D turbulence (Re = 1e4, del^8 hyperviscosity nu4 = 1e-16 as an
+integrating factor, drag 0.1, white forcing on 4 <= |k| <= 6 at injection
+rate 1e-3 with seed 42, adaptive dt at CFL 0.5) at 4096^2 (the benchmark
+grid) and 2048^2 (the closure's fine grid).  The GPU starts from the oracle's
+random-phase field (uploaded), so both sides step identical inputs with
+identical forcing draws:
+  FP64: two steps, relative L2 of omega_hat <= 1e-11, t to 1e-13;
+  FP32 stages: one step, relative L2 <= 1e-5 against the FP64 oracle.
+The step being checked is the reference's RK4 step
+(src/spectral/spectral_time_integration.c:34-106) with the integrating
+factor, forcing and drag of the workload; the oracle restates it in FP64 C.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2 * np.pi
+RE, DRAG, RATE, SEED, CFL = 1e4, 0.1, 1e-3, 42, 0.5
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu] {request.node.name}", flush=True)
+
+
+def hv(n):
+    return dict(enabled=1, coefficient=1e-16 * (np.sqrt(2) * n / 2) ** 8, order=8,
+                cutoff_ratio=0.0)
+
+
+_runs = {}
+
+
+def oracle_run(oc, n, steps=2):
+    """Oracle trajectory of the workload: initial field and the state after
+    each step (cached per grid size for the module)."""
+    if n in _runs:
+        return _runs[n]
+    o = oc.Oracle(n, n, TWO_PI, TWO_PI, RE)
+    o.init_grids()
+    o.hv_set(**hv(n))
+    o.d.drag = DRAG
+    o.d.cfl_factor = CFL
+    o.init_random_phase(SEED, 10.0, 1.0)
+    w0 = o.spec("omega_hat").copy()
+    f = oc.OracleForcing(o.k2, n // 2 + 1, n, 4.0, 6.0, energy_target=1.0, seed=SEED)
+    out = []
+    for _ in range(steps):
+        o.d.dt = o.cfl()
+        o.advance_if(o.project_forcing(f.generate(o.d.dt), RATE))
+        out.append((o.spec("omega_hat").copy(), o.d.t, o.d.dt))
+    f.close()
+    o.close()
+    _runs[n] = (w0, out)
+    return _runs[n]
+
+
+def gpu_solver(ns, n, fp32, w0):
+    s = ns.Spectral(n, n, TWO_PI, TWO_PI, RE)
+    s.init(taylor_green=False)
+    s.hv_set(**hv(n))
+    s.set_precision(fp32)
+    s.set_scheme(True)
+    s.set_drag(DRAG)
+    s.set_forcing(4.0, 6.0, RATE, seed=SEED)
+    s.d.cfl_factor = CFL
+    s.spec("omega_hat")[:] = w0
+    s.d.t = 0.0
+    s.sync_device()
+    return s
+
+
+def download(ns, s):
+    out = np.zeros((s.ny, s.nkx), np.complex128)
+    s.check(s.L.nsd_state_download(s.ctx, ns._dp(out)))
+    return out
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_forced_workload_fp64_vs_oracle(ns, oc, n):
+    w0, ref = oracle_run(oc, n)
+    # the device's own random-phase field is the oracle's
+    g = ns.Spectral(n, n, TWO_PI, TWO_PI, RE)
+    g.init(taylor_green=False)
+    g.random_phase(SEED, k_peak=10.0, energy=1.0)
+    e0 = rel_l2(g.spec("omega_hat"), w0)
+    g.close()
+    print(f"{n}^2 random-phase field vs oracle: rel L2 {e0:.2e}", flush=True)
+    assert e0 <= 1e-14
+    s = gpu_solver(ns, n, False, w0)
+    for k, (w_ref, t_ref, dt_ref) in enumerate(ref):
+        assert s.advance_n(1, True) == 0
+        w = download(ns, s)
+        err = rel_l2(w, w_ref)
+        print(f"{n}^2 FP64 step {k + 1}: dt {s.d.dt:.6e} (oracle {dt_ref:.6e}) "
+              f"omega_hat rel L2 {err:.2e}", flush=True)
+        assert s.d.dt == pytest.approx(dt_ref, rel=1e-13)
+        assert s.d.t == pytest.approx(t_ref, rel=1e-13)
+        assert err <= 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_forced_workload_fp32_vs_oracle(ns, oc, n):
+    w0, ref = oracle_run(oc, n)
+    s = gpu_solver(ns, n, True, w0)
+    assert s.advance_n(1, True) == 0
+    w = download(ns, s)
+    w_ref, t_ref, dt_ref = ref[0]
+    err = rel_l2(w, w_ref)
+    # the low modes carry the energy; their error is the one the diagnostics see
+    low = np.concatenate([w[:33, :33], w[-32:, :33]]), \
+        np.concatenate([w_ref[:33, :33], w_ref[-32:, :33]])
+    err_low = rel_l2(*low)
+    print(f"{n}^2 FP32 stages, 1 step vs FP64 oracle: omega_hat rel L2 {err:.2e}, "
+          f"|k| <= 32 rel L2 {err_low:.2e}; dt {s.d.dt:.8e} vs {dt_ref:.8e}", flush=True)
+    assert s.d.dt == pytest.approx(dt_ref, rel=1e-6)
+    assert err <= 1e-5 and err_low <= 1e-5
+    s.close()

@@ -432,3 +432,46 @@ def test_device_params_reject_negative_drag(ns):
     p.drag = 0.1
     assert L.nsd_set_params(h, C.byref(p)) == 0
     L.nsd_destroy(h)
+
+
+def test_analyze_convergence_vs_oracle(ns, oc, capfd):
+    """ns_spectral_analyze_convergence (spectral_analysis.c:69-100) through the
+    GPU reduction: the closed-form spectrum of the oracle's pin, then a
+    random field after 10 reference steps against the oracle, and the line it
+    prints."""
+    n = 16
+    s = solver(ns, n)
+    s.init(taylor_green=False)
+    w = s.spec("omega_hat")
+    w[:] = 0
+    w[0, 4] = 3.0   # k^2 = 16 <= k2[last]/4 = 16.25: max, not in the RMS
+    w[1, 4] = 2.0j  # k^2 = 17: the only counted mode of 144 - 29 = 115
+    m = s.convergence_metrics()
+    assert m[2] == pytest.approx(3.0, rel=1e-15)
+    assert m[3] == pytest.approx(np.sqrt(4.0 / 115.0), rel=1e-14)
+    s.close()
+
+    n = 64
+    s = solver(ns, n, Re=500.0)
+    s.init(taylor_green=False)
+    s.random_phase(11, k_peak=8.0)
+    o = oracle(oc, n, Re=500.0)
+    o.init_random_phase(11, k_peak=8.0)
+    for _ in range(10):
+        s.d.dt = s.cfl()
+        o.d.dt = o.cfl()
+        s.advance_rk4()
+        o.advance_rk4()
+    m = s.convergence_metrics()
+    r = o.analyze_convergence()
+    print("gpu", m, "oracle", r)
+    assert np.allclose(m, r, rtol=1e-11, atol=0)
+    capfd.readouterr()
+    s.L.ns_spectral_analyze_convergence(s.p, 7)  # only every 10th step prints
+    s.L.ns_spectral_analyze_convergence(s.p, 10)
+    s.check()
+    out = capfd.readouterr().out
+    line = ("Step 10: t=%.4f, E=%.6e, \u03a9=%.6e, max|\u03c9\u0302|=%.2e, err=%.2e, dt=%.2e"
+            % (r[4], r[0], r[1], r[2], r[3], r[5]))
+    assert out.strip() == line
+    s.close()

@@ -373,3 +373,80 @@ def test_config2_fixture_reproduces_and_decays(oc):
     assert o.d.t == pytest.approx(h[1, 1], rel=1e-13)
     assert o.energy() == pytest.approx(h[1, 3], rel=1e-12)
     assert o.enstrophy() == pytest.approx(h[1, 4], rel=1e-12)
+
+
+def test_analyze_convergence_closed_forms(oc):
+    """spectral_analysis.c:69-100: max|w_hat| over the stored modes and the RMS
+    of |w_hat| over the modes with k^2 > k2[last]/4, where k2[last] is the last
+    stored mode (quirk Q7: (N/2)^2 + 1, not the corner 2 (N/2)^2)."""
+    n = 16
+    o = oc.Oracle(n, n)
+    o.init_grids()
+    w = o.spec("omega_hat")
+    # (a) constant modulus c on every stored mode, arbitrary phases: max = err = c
+    c = 0.37
+    phase = np.exp(1j * np.linspace(0.0, 5.0, w.size)).reshape(w.shape)
+    w[:] = c * phase
+    out = o.analyze_convergence()
+    assert out[2] == pytest.approx(c, rel=1e-15)
+    assert out[3] == pytest.approx(c, rel=1e-15)
+    # (b) two modes straddling the threshold k2[last]/4 = (64 + 1)/4 = 16.25:
+    # (kx, ky) = (4, 0), k^2 = 16, amplitude 3 (max, not counted) and
+    # (4, 1), k^2 = 17, amplitude 2 (counted).  Stored modes: 9 x 16 = 144;
+    # lattice points with kx >= 0 and k^2 <= 16: 9 + 7 + 7 + 5 + 1 = 29.
+    w[:] = 0
+    w[0, 4] = 3.0
+    w[1, 4] = 2.0j
+    out = o.analyze_convergence()
+    assert out[2] == pytest.approx(3.0, rel=1e-15)
+    assert out[3] == pytest.approx(np.sqrt(4.0 / (144 - 29)), rel=1e-14)
+    # a mode below the threshold alone gives zero error
+    w[1, 4] = 0
+    assert o.analyze_convergence()[3] == 0.0
+    # E, Z come from the physical arrays; t and dt are passed through
+    o.init_taylor_green()
+    o.compute_derivatives()
+    o.d.t, o.d.dt = 0.25, 1e-3
+    out = o.analyze_convergence()
+    assert out[0] == pytest.approx(0.25, rel=1e-13) and out[1] == pytest.approx(0.5, rel=1e-13)
+    assert out[4] == 0.25 and out[5] == 1e-3
+
+
+def test_if_rk4_energy_enstrophy_budget_closes(oc):
+    """The oracle's integrating-factor step conserves what the dealiased
+    Jacobian conserves: E(T) - E(0) equals minus the Simpson integral of
+    2 nu Z + 2 mu E + sum w nu4 k^8 |W|^2 / k^2 (and likewise for Z), to the
+    quadrature's O(dt^4) (about 1e-12 here).  Dropping the drag, the
+    hyperviscous or the viscous term misses by more than 1e-4.  At 256^2 the
+    modes of the extra x column the mask keeps (quirk Q1, k ~ 86) hold
+    e^-74 of the energy; at 128^2 (k ~ 43) their aliased transfer already
+    shows at 1e-10 of the budget."""
+    n, steps, dt, Re, mu, nu4 = 256, 20, 2e-4, 1e4, 0.1, 1e-16
+    o = oc.Oracle(n, n, TWO_PI, TWO_PI, Re)
+    o.init_grids()
+    o.hv_set(enabled=1, coefficient=nu4 * (np.sqrt(2) * n / 2) ** 8, order=8, cutoff_ratio=0.0)
+    o.d.drag = mu
+    o.init_random_phase(7, 10.0, 1.0)
+    kx = np.arange(n // 2 + 1, dtype=float)
+    ky = np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n).astype(float)
+    k2 = kx[None, :] ** 2 + ky[:, None] ** 2
+    wt = np.where(kx == 0, 1.0, 2.0)[None, :]
+    inv_k2 = np.where(k2 > 0, 1.0 / np.maximum(k2, 1e-300), 0.0)
+    rate = k2 / Re + mu + nu4 * k2 ** 4
+
+    def terms():
+        a2 = np.abs(o.spec("omega_hat")) ** 2
+        return (0.5 * np.sum(wt * a2 * inv_k2), 0.5 * np.sum(wt * a2),
+                -np.sum(wt * rate * a2 * inv_k2), -np.sum(wt * rate * a2))
+
+    hist = [terms()]
+    for _ in range(steps):
+        o.d.dt = dt
+        o.advance_if()
+        hist.append(terms())
+    E, Z, dE, dZ = (np.array(c) for c in zip(*hist))
+    simpson = np.ones(steps + 1)
+    simpson[1:-1:2], simpson[2:-1:2] = 4.0, 2.0
+    for X, dX in ((E, dE), (Z, dZ)):
+        integral = dt / 3.0 * np.sum(simpson * dX)
+        assert abs((X[-1] - X[0]) - integral) <= 1e-9 * abs(integral)
