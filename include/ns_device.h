@@ -47,6 +47,16 @@ typedef struct {
     double force_tau;
     double force_rate;      /* energy injection rate of omega_hat += sqrt(dt) f   */
     unsigned long long force_seed;
+    /* adaptive-dt error controller (build-defined; AdaptiveTimeParams of
+     * include/time_integration.h:16-22): with errctl on, adaptive steps take
+     * dt = clamp(min(CFL dt, dt_err), dt_min, dt_max), where dt_err follows
+     * an embedded estimate; a step whose error exceeds the tolerance
+     * err_atol + err_rtol |omega_hat| is rejected and taken again (see
+     * nsd_errctl_stats).  Not with selective frequency damping. */
+    int errctl;
+    double err_rtol, err_atol;
+    double err_safety;      /* AdaptiveTimeParams.safety_factor, in (0, 1]       */
+    double dt_min, dt_max;  /* AdaptiveTimeParams.dt_min, dt_max                 */
 } NsdParams;
 
 NsdContext *nsd_create(int nx, int ny, double Lx, double Ly, int precision, char *err,
@@ -73,8 +83,15 @@ int nsd_get_forcing_counter(NsdContext *c, unsigned long long *calls);
  * to_device != 0 uploads, else downloads */
 int nsd_forcing_ou(NsdContext *c, double *ou, int count, int to_device);
 int nsd_forcing_modes(const NsdContext *c);
+/* nsteps steps (adaptive != 0: CFL dt, and the error controller if on;
+ * with the controller each call ends with the last step judged, so a
+ * rejected step leaves the state and t of the step before it) */
 int nsd_advance(NsdContext *c, int nsteps, int adaptive);
 int nsd_advance_graph(NsdContext *c, int nsteps, int adaptive);
+/* error controller of member `member`: judged steps accepted and rejected,
+ * the last judged error norm (1 = the tolerance) and the step it allows next */
+int nsd_errctl_stats(NsdContext *c, int member, unsigned long long *accepted,
+                     unsigned long long *rejected, double *last_err, double *dt_next);
 int nsd_state_derived(NsdContext *c, double *psi_hat, double *u_hat, double *v_hat, double *u,
                       double *v, double *omega);
 /* out = {E, Z} from the spectral state by Parseval, FP64 accumulation in a
@@ -189,6 +206,21 @@ int nsd_forcing_generate(int spectral_size, const int *active_idx, int nactive,
                          unsigned long long seed_eff, unsigned long long calls, int colored_on,
                          double alpha, double beta, double energy_target, double *colored,
                          double *out, char *err, size_t errlen);
+
+/* ---- allocation-free forward transform.  No counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  The same transform as
+ * nsd_physical_to_spectral -- r2c, x 1/(nx ny), 2/3 mask
+ * (spectral_transforms.c:9-22) -- on device arrays: `field` FP64 [ny][nx],
+ * `field_hat` FP64 interleaved complex [ny][nx/2+1], queued on the HIP
+ * stream `stream` (NULL: the null stream) of the current device, which must
+ * be synchronised before reading `field_hat`.  Needs no context and calls no
+ * allocator: its intermediate field and twiddles live in one 6 MiB device
+ * array per GPU, which bounds nx and ny to powers of two in [16, 1024]
+ * (larger sizes return SOLVER_INVALID_PARAM with a message in `err`).  Calls
+ * that overlap on one GPU share that array and race. */
+int nsd_physical_to_spectral_noalloc(int nx, int ny, const double *field, double *field_hat,
+                                     void *stream, char *err, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@
 #include "common_types.h"
 #include "forcing.h"
 #include "ns_config.h"
+#include "time_integration.h"
 
 #ifdef __cplusplus
 typedef double fftw_complex[2];
@@ -199,6 +200,24 @@ int ns_spectral_advance_n(NSSpectralData *data, int nsteps, int adaptive);
 int ns_spectral_sync_host(NSSpectralData *data);
 /* energy and enstrophy of the device state (Parseval, FP64 sums) */
 int ns_spectral_device_diagnostics(NSSpectralData *data, double *energy, double *enstrophy);
+/* Adaptive-dt error controller for ns_spectral_advance_n(..., adaptive = 1),
+ * with the parameters the reference declares and never uses
+ * (AdaptiveTimeParams, include/time_integration.h:16-22): every step takes
+ * dt = clamp(min(cfl rule with params->cfl_factor, dt_err), dt_min, dt_max)
+ * and computes, in FP64, err = |y4 - y2| / (atol + rtol |y4|) from the RK4
+ * solution y4 and the midpoint solution y2 its own second stage gives
+ * (Lawson midpoint with the integrating factor); err > 1 rejects the step
+ * (state and t return to the step before, unless dt is at dt_min) and the
+ * next steps may use dt_err = dt clamp(safety_factor err^(-1/3), 0.2, 5).
+ * adapt_enabled = 0 (or params == NULL) switches it off.  Not with
+ * selective frequency damping. */
+int ns_spectral_set_error_control(NSSpectralData *data, const AdaptiveTimeParams *params,
+                                  double rtol, double atol);
+/* judged steps accepted and rejected since sync_device, the last error norm
+ * and the step it allows next */
+int ns_spectral_error_control_stats(NSSpectralData *data, unsigned long long *accepted,
+                                    unsigned long long *rejected, double *last_err,
+                                    double *dt_next);
 /* the numbers ns_spectral_analyze_convergence prints (spectral_analysis.c:69-100),
  * from the host arrays: out = {E, Z, max|omega_hat|, spectral error (RMS of
  * |omega_hat| over the modes with k^2 > k2[last] / 4), t, dt} */

@@ -46,6 +46,7 @@ EXPORTED = [
     "ns_spectral_initialize_random_phase", "ns_spectral_sync_device", "ns_spectral_advance_n",
     "ns_spectral_sync_host", "ns_spectral_device_diagnostics", "ns_spectral_configure",
     "ns_spectral_create_from_config", "ns_spectral_convergence_metrics",
+    "ns_spectral_set_error_control", "ns_spectral_error_control_stats",
     # forcing.h
     "ns_forcing_config_default", "ns_forcing_state_init", "ns_forcing_state_free",
     "ns_forcing_generate",
@@ -70,7 +71,7 @@ EXPORTED = [
     "nsd_batch_energy", "nsd_batch_scalars", "nsd_member_upload", "nsd_member_download",
     "nsd_multi_create", "nsd_multi_destroy", "nsd_multi_error", "nsd_multi_advance",
     "nsd_multi_synchronize", "nsd_autotune", "nsd_set_variants", "nsd_forcing_ou",
-    "nsd_forcing_modes",
+    "nsd_forcing_modes", "nsd_physical_to_spectral_noalloc", "nsd_errctl_stats",
     # checkpoint.h
     "ns_checkpoint_write_spectral", "ns_checkpoint_read_spectral", "ns_checkpoint_error",
     # ns_closure.h
@@ -151,7 +152,9 @@ class NsdParams(C.Structure):  # ns_device.h
                 ("force_on", C.c_int), ("force_colored", C.c_int), ("force_kmin", C.c_double),
                 ("force_kmax", C.c_double), ("force_energy_target", C.c_double),
                 ("force_tau", C.c_double), ("force_rate", C.c_double),
-                ("force_seed", C.c_ulonglong)]
+                ("force_seed", C.c_ulonglong), ("errctl", C.c_int), ("err_rtol", C.c_double),
+                ("err_atol", C.c_double), ("err_safety", C.c_double), ("dt_min", C.c_double),
+                ("dt_max", C.c_double)]
 
 
 class NSEnsembleConfig(C.Structure):  # ns_ensemble.h
@@ -212,6 +215,11 @@ def load():
                      ("ns_spectral_advance_n", [C.c_int, C.c_int]),
                      ("ns_spectral_device_diagnostics", [c_double_p, c_double_p]),
                      ("ns_spectral_convergence_metrics", [c_double_p]),
+                     ("ns_spectral_set_error_control",
+                      [C.POINTER(AdaptiveTimeParams), C.c_double, C.c_double]),
+                     ("ns_spectral_error_control_stats",
+                      [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong), c_double_p,
+                       c_double_p]),
                      ("ns_spectral_configure", [C.POINTER(NSConfig)])):
         getattr(L, n).argtypes = [P] + extra
         getattr(L, n).restype = C.c_int
@@ -322,6 +330,9 @@ def load():
         "nsd_set_variants": ([V, C.c_int, C.c_int], C.c_int),
         "nsd_forcing_ou": ([V, D, C.c_int, C.c_int], C.c_int),
         "nsd_forcing_modes": ([V], C.c_int),
+        "nsd_physical_to_spectral_noalloc": ([C.c_int, C.c_int, V, V, V, C.c_char_p,
+                                              C.c_size_t], C.c_int),
+        "nsd_errctl_stats": ([V, C.c_int, ULLP, ULLP, D, D], C.c_int),
         "ns_checkpoint_write_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
         "ns_checkpoint_read_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
         "ns_checkpoint_error": ([], C.c_char_p),
@@ -531,6 +542,20 @@ class Spectral:
         e, z = C.c_double(), C.c_double()
         self.check(self.L.ns_spectral_device_diagnostics(self.p, C.byref(e), C.byref(z)))
         return e.value, z.value
+
+    def set_error_control(self, rtol, atol=0.0, safety=0.9, dt_min=1e-8, dt_max=0.1,
+                          cfl_factor=None, enabled=True):
+        p = AdaptiveTimeParams(self.d.cfl_factor if cfl_factor is None else cfl_factor,
+                               dt_min, dt_max, safety, 1 if enabled else 0)
+        self.check(self.L.ns_spectral_set_error_control(self.p, C.byref(p), rtol, atol))
+
+    def error_control_stats(self):
+        """(accepted, rejected, last error norm, next step limit)"""
+        a, r = C.c_ulonglong(), C.c_ulonglong()
+        e, dtn = C.c_double(), C.c_double()
+        self.check(self.L.ns_spectral_error_control_stats(self.p, C.byref(a), C.byref(r),
+                                                          C.byref(e), C.byref(dtn)))
+        return a.value, r.value, e.value, dtn.value
 
     def convergence_metrics(self):
         """{E, Z, max|omega_hat|, spectral error, t, dt} of analyze_convergence"""

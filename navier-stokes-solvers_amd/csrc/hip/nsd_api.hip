@@ -273,14 +273,73 @@ __global__ void k_d2f(size_t n, const double* a, float* out) {
 
 // time, dt and forcing counter of every member
 __global__ void k_set_scalars(DevScalars* sc, int nb, int which, double t, double dt,
-                              unsigned long long fcount) {
+                              unsigned long long fcount, double dt_err) {
     for (int m = threadIdx.x; m < nb; m += blockDim.x) {
         if (which & 1) {
             sc[m].t = t;
             sc[m].dt = dt;
         }
         if (which & 2) sc[m].fcount = fcount;
+        if (which & 4) {  // error controller: a fresh start
+            sc[m].dt_err = dt_err;
+            sc[m].err = 0.0;
+            sc[m].n_acc = sc[m].n_rej = 0ull;
+            sc[m].pending = sc[m].redo = 0;
+        }
     }
+}
+
+// Error controller, at the start of every step (one workgroup per member):
+// judge the candidate the previous step left.  err = |y4 - y2| / (atol +
+// rtol |y4|) from the last stage's workgroup partials, summed in index order;
+// err <= 1 (or a step already at dt_min) accepts, else t goes back by the
+// candidate's dt and this step restores the state kept before it (redo).
+// Either way the next steps may use dt_err = dt clamp(safety err^(-1/3),
+// 0.2, 5): the estimate y4 - y2 is O(dt^3).
+__global__ void k_judge(DevScalars* sc, const double* part, size_t stride, Phys ph) {
+    DevScalars* s = sc + blockIdx.x;
+    const double* p = part + blockIdx.x * stride;
+    __shared__ double re[256], rz[256];
+    const int pending = s->pending;
+    const int n = s->errparts;
+    double e = 0.0, z = 0.0;
+    if (pending)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            e += p[2 * i];
+            z += p[2 * i + 1];
+        }
+    re[threadIdx.x] = e;
+    rz[threadIdx.x] = z;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            re[threadIdx.x] += re[threadIdx.x + o];
+            rz[threadIdx.x] += rz[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (!pending) {
+        s->redo = 0;
+        return;
+    }
+    const double num = sqrt(re[0]), den = ph.err_atol + ph.err_rtol * sqrt(rz[0]);
+    const double err = den > 0.0 ? num / den : (num > 0.0 ? __builtin_inf() : 0.0);
+    const double fac =
+        err > 0.0 ? fmin(kErrFacMax, fmax(kErrFacMin, ph.err_safety * pow(err, -1.0 / 3.0)))
+                  : kErrFacMax;
+    const double dt = s->dt;
+    s->err = err;
+    s->pending = 0;
+    if (err <= 1.0 || dt <= ph.dt_lo) {
+        s->n_acc++;
+        s->redo = 0;
+    } else {
+        s->n_rej++;
+        s->t -= dt;
+        s->redo = 1;
+    }
+    s->dt_err = dt * fac;
 }
 
 // Parseval sums over the compact state: E = 1/2 sum_full |w|^2/k^2,
@@ -426,10 +485,12 @@ struct ForceArgs {
     void* fbox;
     DevScalars* sc;
     const unsigned long long* m_seed;  // per-member seeds of a batch (else seed)
+    int skip_on_redo;                  // error controller: no draw on a restoring step
 };
 
 __global__ void k_force(ForceArgs a) {
     const Geo& G = a.geo;
+    if (a.skip_on_redo && a.sc[blockIdx.y].redo) return;
     {  // member blockIdx.y of a batch: its own generator, OU state, box and scalars
         const int mb = blockIdx.y;
         const size_t boxn = (size_t)a.fbx * (2 * a.fby + 1);
@@ -639,6 +700,9 @@ struct NsdContext {
     void* cl_m = nullptr;                   // closure: mixed-layout scratch
     float* cl_y = nullptr;                  // closure: physical tendency
     void* cl_src = nullptr;                 // closure: compact spectral tendency
+    void* ec_prev = nullptr;                // error controller: state before the step
+    void* ec_ymid = nullptr;                // error controller: midpoint solution
+    double* ec_part = nullptr;              // error controller: workgroup partials
     NsdParams P;
     Phys phys;
     hipStream_t stream = nullptr;
@@ -752,6 +816,10 @@ ColArgs<T> col_args(NsdContext* c) {
     a.m_inv_re = c->m_inv_re;
     a.src = c->closure ? (const cx<T>*)c->cl_src : nullptr;
     a.pairs = pairs_ok(c) ? 1 : 0;
+    a.prev = (cx<T>*)c->ec_prev;
+    a.ymid = (cx<T>*)c->ec_ymid;
+    a.errpart = c->ec_part;
+    a.bp = 2 * (size_t)c->G.nkx_keep;
     return a;
 }
 
@@ -779,10 +847,10 @@ void launch_col_nt(NsdContext* c, const ColArgs<T>& a, int blocks, size_t lds) {
 }
 
 template <typename T>
-void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols) {
+void launch_col_t(NsdContext* c, const ColArgs<T>& a, int ncols, bool errctl = false) {
     const int L = c->G.ny;
     if (a.mode == COL_STAGE || a.mode == COL_PRIME) {
-        launch_col_fused<T>(a, L, ncols, c->nb, c->vcol, c->stream);
+        launch_col_fused<T>(a, L, ncols, c->nb, c->vcol, c->stream, errctl);
         return;
     }
     const int NT = threads_for(L);
@@ -804,10 +872,10 @@ void launch_row_nt(NsdContext* c, const RowArgs<T>& a, int blocks, size_t lds) {
 }
 
 template <typename T>
-void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items) {
+void launch_row_t(NsdContext* c, const RowArgs<T>& a, int items, bool errctl = false) {
     const int L = c->G.nx;
     if (a.mode == ROW_STAGE) {
-        launch_row_stage<T>(a, L, items, c->nb, c->vrow, c->stream);
+        launch_row_stage<T>(a, L, items, c->nb, c->vrow, c->stream, errctl);
         return;
     }
     const int NT = threads_for(L);
@@ -859,11 +927,16 @@ int op_prime(NsdContext* c) {
     return lastk(c);
 }
 
+// the error controller runs with adaptive steps only
+bool errctl_on(const NsdContext* c, int adaptive) { return c->phys.errctl && adaptive; }
+
 template <typename T>
 void enqueue_stage(NsdContext* c, int s, int adaptive) {
+    const bool ec = errctl_on(c, adaptive);
     RowArgs<T> r = row_args<T>(c);
     r.mode = ROW_STAGE;
     r.track_umax = (s == 0) ? 1 : 0;
+    r.skip_on_redo = (s > 0) ? 1 : 0;
     r.stagger_lo = c->ncu;  // second resident workgroup of each CU
     r.stagger_hi = 2 * c->ncu;
     ColArgs<T> a = col_args<T>(c);
@@ -876,10 +949,30 @@ void enqueue_stage(NsdContext* c, int s, int adaptive) {
     const bool prof = c->prof != 0;
     const size_t e0 = (size_t)c->prof_steps * kEvPerStep + 3 * s;
     if (prof) (void)hipEventRecord(c->evs[e0], c->stream);
-    launch_row_t<T>(c, r, c->G.ny / 2);
+    launch_row_t<T>(c, r, c->G.ny / 2, ec);
     if (prof) (void)hipEventRecord(c->evs[e0 + 1], c->stream);
-    launch_col_t<T>(c, a, c->G.nkx_keep);
+    launch_col_t<T>(c, a, c->G.nkx_keep, ec);
     if (prof) (void)hipEventRecord(c->evs[e0 + 2], c->stream);
+}
+
+// error controller: the judge of the previous candidate (start of a step)
+void enqueue_judge(NsdContext* c) {
+    hipLaunchKernelGGL(k_judge, dim3(c->nb), dim3(256), 0, c->stream, c->sc,
+                       (const double*)c->ec_part, 2 * (size_t)c->G.nkx_keep, c->phys);
+}
+
+// error controller, after the last step of a call: judge the last candidate
+// and, if it is rejected, restore the state before it (stage-0 column pass in
+// finalize mode), so that the state between calls is always an accepted one
+template <typename T>
+void enqueue_finalize(NsdContext* c) {
+    enqueue_judge(c);
+    ColArgs<T> a = col_args<T>(c);
+    a.mode = COL_STAGE;
+    a.stage = 0;
+    a.finalize = 1;
+    a.phys.adaptive = 1;
+    launch_col_t<T>(c, a, c->G.nkx_keep, true);
 }
 
 template <typename T>
@@ -909,6 +1002,7 @@ void enqueue_force(NsdContext* c, hipStream_t st) {
     f.fbox = c->fbox;
     f.sc = c->sc;
     f.m_seed = c->m_seed;
+    f.skip_on_redo = c->phys.errctl ? 1 : 0;
     hipLaunchKernelGGL(k_force, dim3(1, c->nb), dim3(256), 0, st, f);
 }
 
@@ -963,6 +1057,7 @@ void enqueue_closure(NsdContext* c) {
 // and the forcing counter).
 template <typename T>
 void enqueue_step(NsdContext* c, int adaptive) {
+    if (errctl_on(c, adaptive)) enqueue_judge(c);
     if (c->closure) enqueue_closure<T>(c);
     const bool side = force_active<T>(c);
     for (int s = 0; s < 4; ++s) {
@@ -994,6 +1089,7 @@ int op_advance(NsdContext* c, int nsteps, int adaptive) {
         enqueue_step<T>(c, adaptive);
         if (c->prof) c->prof_steps++;
     }
+    if (errctl_on(c, adaptive) && nsteps > 0) enqueue_finalize<T>(c);
     if (c->prof) {
         CK(hipEventRecord(c->evs[last + 1], c->stream));
         CK(hipStreamSynchronize(c->stream));
@@ -1040,6 +1136,10 @@ int op_advance_graph(NsdContext* c, int nsteps, int adaptive) {
         c->g_gen = c->gen;
     }
     for (int n = 0; n < nsteps; ++n) CK(hipGraphLaunch(c->gexec, c->stream));
+    if (errctl_on(c, adaptive) && nsteps > 0) {
+        enqueue_finalize<T>(c);
+        return lastk(c);
+    }
     return ST_OK;
 }
 
@@ -1447,6 +1547,13 @@ const char* nsd_multi_error(const NsdMulti* mh) { return mh ? mh->err.c_str() : 
 int nsd_multi_advance(NsdMulti* mh, int nsteps, int adaptive) {
     if (!mh || nsteps < 0) return ST_PARAM;
     adaptive = adaptive ? 1 : 0;
+    for (NsdContext* c : mh->cs)
+        if (errctl_on(c, adaptive)) {
+            // its finalize pass would have to follow the whole graph on the
+            // context's own stream
+            mh->err = "the error controller runs on single contexts, not in multi-size graphs";
+            return ST_PARAM;
+        }
     bool stale = !mh->gexec || mh->adaptive != adaptive;
     for (size_t i = 0; i < mh->cs.size(); ++i) stale = stale || mh->gens[i] != mh->cs[i]->gen;
     if (stale && multi_capture(mh, adaptive)) return ST_MEM;
@@ -1579,7 +1686,7 @@ void nsd_destroy(NsdContext* c) {
                     c->mixed[0], c->mixed[2], c->stH[0], c->stH[1],
                     c->stH[2], c->stH[3], c->stR[0], c->stR[1], c->stR[2], c->stR[3], c->fmodes,
                     c->fpartner, c->fou, c->fwork, c->fbox, c->m_inv_re, c->m_seed, c->m_ic,
-                    c->cl_m, c->cl_y, c->cl_src};
+                    c->cl_m, c->cl_y, c->cl_src, c->ec_prev, c->ec_ymid, c->ec_part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->fstream) (void)hipStreamSynchronize(c->fstream);
@@ -1607,6 +1714,26 @@ int nsd_set_params(NsdContext* c, const NsdParams* p) {
         c->err = "drag and the hyperviscosity coefficient must not be negative";
         return ST_PARAM;
     }
+    if (p->errctl) {
+        if (!(p->err_rtol >= 0.0) || !(p->err_atol >= 0.0) ||
+            !(p->err_rtol > 0.0 || p->err_atol > 0.0)) {
+            c->err = "error tolerances must be non-negative and not both zero";
+            return ST_PARAM;
+        }
+        if (!(p->err_safety > 0.0 && p->err_safety <= 1.0)) {
+            c->err = "error controller safety factor must be in (0, 1]";
+            return ST_PARAM;
+        }
+        if (!(p->dt_min > 0.0) || !(p->dt_max >= p->dt_min)) {
+            c->err = "error controller needs 0 < dt_min <= dt_max";
+            return ST_PARAM;
+        }
+        if (p->sfd_on) {
+            c->err = "the error controller does not keep the SFD state of a rejected step; "
+                     "switch selective frequency damping off";
+            return ST_PARAM;
+        }
+    }
     c->P = *p;
     Phys& f = c->phys;
     f.inv_Re = 1.0 / p->Re;
@@ -1625,6 +1752,27 @@ int nsd_set_params(NsdContext* c, const NsdParams* p) {
     f.dxmin = fmin(c->G.Lx / c->G.nx, c->G.Ly / c->G.ny);
     f.force_on = 0;
     c->gen++;
+    {  // error controller: buffers on first use, fresh scalars when switched on
+        const int was_on = f.errctl;
+        f.errctl = p->errctl ? 1 : 0;
+        f.err_rtol = p->err_rtol;
+        f.err_atol = p->err_atol;
+        f.err_safety = p->err_safety;
+        f.dt_lo = p->errctl ? p->dt_min : 1e-8;
+        f.dt_hi = p->errctl ? p->dt_max : 0.1;
+        if (f.errctl) {
+            const size_t C = C_of(c) * c->esz * c->nb;
+            if (!c->ec_prev) CK(hipMalloc(&c->ec_prev, C));
+            if (!c->ec_ymid) CK(hipMalloc(&c->ec_ymid, C));
+            if (!c->ec_part)
+                CK(hipMalloc((void**)&c->ec_part, 2 * sizeof(double) * c->G.nkx_keep * c->nb));
+            if (!was_on) {
+                hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb, 4,
+                                   0.0, 0.0, 0ull, f.dt_hi);
+                if (lastk(c) || sync(c)) return ST_MEM;
+            }
+        }
+    }
     // forcing: active modes in reference order (forcing.c:80-93)
     for (void** b : {(void**)&c->fmodes, (void**)&c->fpartner, (void**)&c->fou,
                      (void**)&c->fwork, &c->fbox}) {
@@ -1713,8 +1861,10 @@ int nsd_state_random_phase(NsdContext* c, unsigned long long seed, double k_peak
 
 int nsd_set_time(NsdContext* c, double t, double dt) {
     if (!c) return ST_PARAM;
-    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb, 1, t, dt,
-                       0ull);
+    // a new time also restarts the error controller (first step limited by
+    // dt_max and the CFL rule alone)
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb, 1 | 4, t,
+                       dt, 0ull, c->phys.dt_hi);
     if (lastk(c)) return ST_MEM;
     return sync(c);
 }
@@ -1733,7 +1883,7 @@ int nsd_get_time(NsdContext* c, double* t, double* dt, int* status) {
 int nsd_set_forcing_counter(NsdContext* c, unsigned long long calls) {
     if (!c) return ST_PARAM;
     hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(256), 0, c->stream, c->sc, c->nb, 2, 0.0, 0.0,
-                       calls);
+                       calls, 0.0);
     if (lastk(c)) return ST_MEM;
     return sync(c);
 }
@@ -1758,6 +1908,19 @@ int nsd_forcing_ou(NsdContext* c, double* ou, int count, int to_device) {
 }
 
 int nsd_forcing_modes(const NsdContext* c) { return c ? c->fA : 0; }
+
+int nsd_errctl_stats(NsdContext* c, int member, unsigned long long* accepted,
+                     unsigned long long* rejected, double* last_err, double* dt_next) {
+    if (!c || member < 0 || member >= c->nb) return ST_PARAM;
+    DevScalars s;
+    CK(hipMemcpyAsync(&s, c->sc + member, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    if (sync(c)) return ST_MEM;
+    if (accepted) *accepted = s.n_acc;
+    if (rejected) *rejected = s.n_rej;
+    if (last_err) *last_err = s.err;
+    if (dt_next) *dt_next = s.dt_err;
+    return ST_OK;
+}
 
 int nsd_advance(NsdContext* c, int nsteps, int adaptive) {
     if (!c || nsteps < 0) return ST_PARAM;

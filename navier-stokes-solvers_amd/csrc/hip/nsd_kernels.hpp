@@ -52,7 +52,14 @@ struct Phys {
     double cfl, dxmin;
     int force_on;
     int fbx, fby;  // forcing box: kx in [0, fbx), ky in [-fby, fby]
+    // adaptive-dt error controller (with `adaptive`; the parameters of
+    // AdaptiveTimeParams, include/time_integration.h:16-22, plus tolerances)
+    int errctl;
+    double err_rtol, err_atol, err_safety, dt_lo, dt_hi;
 };
+
+// growth limits of the error controller's step factor
+constexpr double kErrFacMin = 0.2, kErrFacMax = 5.0;
 
 struct DevScalars {
     double t;
@@ -63,6 +70,14 @@ struct DevScalars {
     unsigned long long redmax[4];   // reduction outputs (max, as ordered bits)
     int status;                     // 0 ok, 2 diverged (SolverStatus values)
     int pad_;
+    // adaptive-dt error controller
+    double dt_err;                  // step the last judged error allows next
+    double err;                     // last judged error norm (1 = tolerance)
+    unsigned long long n_acc, n_rej;  // judged steps accepted / rejected
+    int pending;                    // a candidate step waits to be judged
+    int redo;                       // this step restores the last accepted state
+    int errparts;                   // workgroup partials the candidate left
+    int pad2_;
 };
 
 // ---------------------------------------------------------------- helpers
@@ -386,6 +401,14 @@ struct ColArgs {
     cx<T>* out_c;      // COL_FWD: compact output of the kept modes (instead of out_ref)
     const cx<T>* src;  // COL_STAGE: compact source added to the nonlinear term
     int nbat;          // members along grid y for the transform modes (0 = 1)
+    // error controller (COL_STAGE, ERR): state before the step, midpoint
+    // solution, per-workgroup {sum w |y4 - y2|^2, sum w |y4|^2} (stride bp
+    // per member); finalize: stage 0 only restores a rejected step
+    cx<T>* prev;
+    cx<T>* ymid;
+    double* errpart;
+    size_t bp;
+    int finalize;
 };
 
 template <typename T>
@@ -450,7 +473,7 @@ struct TwSource<true, L, T> {
     using type = TwRegs<L, T>;
 };
 
-template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1>
+template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1, bool ERR = false>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     // a pair-layout length must instantiate the paired writer (the host
     // selects the layout from the grid alone)
@@ -472,6 +495,17 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     DevScalars* const sc = a.sc + mb;
     const cx<T>* const src = a.src ? a.src + mb * a.bc : nullptr;
     const size_t mixed_off = mb * a.bm;
+    // Error controller: a step that restores a rejected candidate runs only
+    // its stage-0 column pass, which writes the restored state and its
+    // derivative fields; a finalize pass does the same, and nothing when the
+    // last candidate was accepted.  (`redo` is uniform over the grid.)
+    int redo = 0;
+    if constexpr (ERR && MODE == COL_STAGE) {
+        redo = sc->redo;
+        if (redo ? a.stage > 0 : a.finalize != 0) return;
+    }
+    cx<T>* const prev = ERR ? a.prev + mb * a.bc : nullptr;
+    cx<T>* const ymid = ERR ? a.ymid + mb * a.bc : nullptr;
     Phys ph = a.phys;
     if (a.m_inv_re) {
         ph.inv_Re = a.m_inv_re[mb];
@@ -557,7 +591,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         if (a.stage == 0 && ph.adaptive) {
             const double u2 = __longlong_as_double((long long)sc->umax2_bits);
             dt = cfl_dt(ph, sqrt(u2));
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+            // the error controller's step limit, within [dt_min, dt_max]
+            if constexpr (ERR) dt = fmin(fmax(fmin(dt, sc->dt_err), ph.dt_lo), ph.dt_hi);
+            if (blockIdx.x == 0 && threadIdx.x == 0 && !redo) {
                 sc->dt = dt;
                 if (!(u2 <= 1.0e300)) sc->status = 2;  // SOLVER_DIVERGED
             }
@@ -569,6 +605,9 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
     const T norm = T(1.0 / ((double)G.nx * (double)G.ny));
     const double sfd_alpha = (ph.sfd_tau > 0.0) ? fmin(fmax(dt / ph.sfd_tau, 0.0), 1.0) : 0.0;
     const T sqh = T(sqrt(dt));
+    // error controller, last stage: this thread's sum w |y4 - y2|^2 and
+    // sum w |y4|^2 (FP64)
+    double esum = 0.0, ssum = 0.0;
 
 #pragma unroll
     for (int s = 0; s < P; ++s) {
@@ -588,10 +627,20 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : a.in0[mb * a.bc + ci];
             continue;
         }
+        if constexpr (ERR && MODE == COL_STAGE) {
+            if (redo) {  // the state of the last accepted step comes back
+                w[s] = prev[ci];
+                state[ci] = w[s];
+                continue;
+            }
+        }
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
         cx<T> N = (-norm) * v[s];
         if (src) N = N + src[ci];  // learned subgrid tendency, frozen over the step
         const cx<T> w0 = PF ? pw0[s] : state[ci];
+        if constexpr (ERR) {
+            if (a.stage == 0) prev[ci] = w0;
+        }
         const T k2 = kx * kx + ky * ky;
         cx<T> wn;
         if constexpr (SCHEME == 0) {
@@ -608,6 +657,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             } else if (a.stage == 1) {
                 acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
+                if constexpr (ERR) ymid[ci] = w0 + h * ks;  // explicit midpoint solution
             } else if (a.stage == 2) {
                 acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * ks;
                 wn = w0 + h * ks;
@@ -633,6 +683,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             } else if (a.stage == 1) {
                 acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e2) + (T(0.5) * h) * N;
+                // Lawson midpoint solution E w0 + h E2 N(b)
+                if constexpr (ERR) ymid[ci] = grow(w0, e1) + h * grow(N, e2);
             } else if (a.stage == 2) {
                 acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e1) + h * grow(N, e2);
@@ -641,6 +693,12 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
         }
         if (a.stage == 3) {
+            if constexpr (ERR) {  // embedded estimate, before the forcing kick
+                const cx<T> e = wn - ymid[ci];
+                const double wt = kxi == 0 ? 1.0 : 2.0;
+                esum += wt * ((double)e.x * (double)e.x + (double)e.y * (double)e.y);
+                ssum += wt * ((double)wn.x * (double)wn.x + (double)wn.y * (double)wn.y);
+            }
             if (ph.force_on && kxi < ph.fbx) {
                 const int b = band_of(G, jy);
                 if (b >= -ph.fby && b <= ph.fby)
@@ -654,8 +712,42 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
         if (MODE == COL_STAGE && a.stage == 3) {
             sc->t += dt;
             sc->umax2_bits = 0ull;
+        } else if (MODE == COL_STAGE && redo) {
+            sc->umax2_bits = 0ull;  // stage 3 does not run on a restoring step
         } else if (MODE == COL_PRIME) {
             sc->umax2_bits = 0ull;
+            sc->pending = 0;  // a new state: nothing waits to be judged
+            sc->redo = 0;
+        }
+    }
+    if constexpr (ERR && MODE == COL_STAGE) {
+        if (a.stage == 3) {
+            // workgroup sums in a fixed order (wave butterflies, then the
+            // waves in index order): the judge adds the partials in index
+            // order, so the decision is reproducible bit for bit
+            __shared__ double ered[2 * (NT / 64)];
+            esum = wave_sum(esum);
+            ssum = wave_sum(ssum);
+            const int wv = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) {
+                ered[2 * wv] = esum;
+                ered[2 * wv + 1] = ssum;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double e = 0.0, z = 0.0;
+                for (int i = 0; i < NT / 64; ++i) {
+                    e += ered[2 * i];
+                    z += ered[2 * i + 1];
+                }
+                double* part = a.errpart + mb * a.bp + 2 * (size_t)blockIdx.x;
+                part[0] = e;
+                part[1] = z;
+                if (blockIdx.x == 0) {
+                    sc->pending = 1;
+                    sc->errparts = (int)gridDim.x;
+                }
+            }
         }
     }
 
@@ -833,6 +925,7 @@ struct RowArgs {
     unsigned short* out_bf16;  // ROW_C2R: field 0 as bf16, times out_bf16_scale (if non-null)
     float out_bf16_scale;
     int pairs;                 // ROW_STAGE: mixed fields in the pair layout (pair_layout)
+    int skip_on_redo;          // ROW_STAGE, ERR: idle when the step restores a rejected one
 };
 
 // Build the Hermitian extension of Z = A + iB in LDS for one row y from
@@ -1079,11 +1172,16 @@ constexpr bool row_prefetch() {
     return LC >= 64 && (LC != 1024 || sizeof(T) == 4);
 }
 
-template <typename T, int NT, int MODE, int LC, int PR = -1>
+template <typename T, int NT, int MODE, int LC, int PR = -1, bool ERR = false>
 __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
     static_assert(MODE != ROW_STAGE || !pair_layout<T, LC>() ||
                       (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()),
                   "pair layout needs the paired gather-ahead k_row path");
+    // error controller: stages 1-3 of a step that restores a rejected
+    // candidate have nothing to do (uniform over the grid)
+    if constexpr (ERR && MODE == ROW_STAGE) {
+        if (a.skip_on_redo && a.sc[blockIdx.y].redo) return;
+    }
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     __shared__ double red[NT / 64];

@@ -43,7 +43,8 @@ size_t var_lds() {
     return (size_t)lines * line_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
 }
 
-template <typename T, int LOG, int V>
+// ERR: the error controller's instantiations (launch variant 0 only)
+template <typename T, int LOG, int V, bool ERR = false>
 void row_stage_LV(const RowArgs<T>& a, int items, int members, hipStream_t st) {
     constexpr int L = 1 << LOG;
     if constexpr (var_ok<L, V>()) {
@@ -52,12 +53,12 @@ void row_stage_LV(const RowArgs<T>& a, int items, int members, hipStream_t st) {
         constexpr int lines = NT * P / L;
         const int blocks = (items + lines - 1) / lines;
         const size_t lds = var_lds<T, L, V>();
-        hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L, PR>), dim3(blocks, members), dim3(NT), lds,
-                           st, a);
+        hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L, PR, ERR>), dim3(blocks, members), dim3(NT),
+                           lds, st, a);
     }
 }
 
-template <typename T, int LOG, int V>
+template <typename T, int LOG, int V, bool ERR = false>
 void col_fused_LV(const ColArgs<T>& a, int ncols, int members, hipStream_t st) {
     constexpr int L = 1 << LOG;
     if constexpr (var_ok<L, V>()) {
@@ -70,9 +71,11 @@ void col_fused_LV(const ColArgs<T>& a, int ncols, int members, hipStream_t st) {
         if (a.mode == COL_PRIME)
             hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L, PR>), grid, dim3(NT), lds, st, a);
         else if (a.phys.scheme == 0)
-            hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 0, L, PR>), grid, dim3(NT), lds, st, a);
+            hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 0, L, PR, ERR>), grid, dim3(NT), lds, st,
+                               a);
         else
-            hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 1, L, PR>), grid, dim3(NT), lds, st, a);
+            hipLaunchKernelGGL((k_col<T, NT, COL_STAGE, 1, L, PR, ERR>), grid, dim3(NT), lds, st,
+                               a);
     }
 }
 
@@ -92,6 +95,11 @@ void limits_LV(int bytes) {
         lds_attr(k_col<T, NT, COL_PRIME, 0, L, PR>, bytes);
         lds_attr(k_col<T, NT, COL_STAGE, 0, L, PR>, bytes);
         lds_attr(k_col<T, NT, COL_STAGE, 1, L, PR>, bytes);
+        if constexpr (V == 0) {
+            lds_attr(k_row<T, NT, ROW_STAGE, L, PR, true>, bytes);
+            lds_attr(k_col<T, NT, COL_STAGE, 0, L, PR, true>, bytes);
+            lds_attr(k_col<T, NT, COL_STAGE, 1, L, PR, true>, bytes);
+        }
     }
 }
 
@@ -115,8 +123,15 @@ int log2i(int L) {
 
 template <typename T>
 void launch_row_stage(const RowArgs<T>& a, int L, int items, int members, int variant,
-                      hipStream_t st) {
+                      hipStream_t st, bool errctl) {
     const int lg = log2i(L);
+    if (errctl) {
+#define E_(n) \
+    if (lg == n) return row_stage_LV<T, n, 0, true>(a, items, members, st);
+        NSD_FOR_LOGS(E_)
+#undef E_
+        return;
+    }
 #define V_(n, v) \
     if (lg == n && variant == v) return row_stage_LV<T, n, v>(a, items, members, st);
 #define X(n) NSD_FOR_VARS(V_, n)
@@ -127,8 +142,15 @@ void launch_row_stage(const RowArgs<T>& a, int L, int items, int members, int va
 
 template <typename T>
 void launch_col_fused(const ColArgs<T>& a, int L, int ncols, int members, int variant,
-                      hipStream_t st) {
+                      hipStream_t st, bool errctl) {
     const int lg = log2i(L);
+    if (errctl && a.mode == COL_STAGE) {
+#define E_(n) \
+    if (lg == n) return col_fused_LV<T, n, 0, true>(a, ncols, members, st);
+        NSD_FOR_LOGS(E_)
+#undef E_
+        return;
+    }
 #define V_(n, v) \
     if (lg == n && variant == v) return col_fused_LV<T, n, v>(a, ncols, members, st);
 #define X(n) NSD_FOR_VARS(V_, n)
@@ -158,8 +180,20 @@ void stage_lds_limits(int bytes) {
 #undef V_
 }
 
-template void launch_row_stage<NSD_T>(const RowArgs<NSD_T>&, int, int, int, int, hipStream_t);
-template void launch_col_fused<NSD_T>(const ColArgs<NSD_T>&, int, int, int, int, hipStream_t);
+template void launch_row_stage<NSD_T>(const RowArgs<NSD_T>&, int, int, int, int, hipStream_t,
+                                      bool);
+template void launch_col_fused<NSD_T>(const ColArgs<NSD_T>&, int, int, int, int, hipStream_t,
+                                      bool);
+
+// workgroups of the default column launch over ncols columns of length L
+// (the number of partials the error controller's last stage leaves)
+template <typename T>
+int col_fused_blocks(int L, int ncols) {
+    const int NT = L / P > 256 ? L / P : 256;
+    const int lines = NT * P / L;
+    return (ncols + lines - 1) / lines;
+}
+template int col_fused_blocks<NSD_T>(int, int);
 template int stage_variant_valid<NSD_T>(int, int, int);
 template void stage_lds_limits<NSD_T>(int);
 

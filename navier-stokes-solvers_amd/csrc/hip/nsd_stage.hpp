@@ -11,13 +11,17 @@ constexpr int kStageVariants = 5;
 
 // k_row in ROW_STAGE mode for rows of length L (power of two, 16..8192),
 // `members` batch members along grid y
+// (errctl: the error controller's instantiation, launch variant 0)
 template <typename T>
 void launch_row_stage(const RowArgs<T>& a, int L, int items, int members, int variant,
-                      hipStream_t st);
+                      hipStream_t st, bool errctl = false);
 // k_col in COL_STAGE (scheme from a.phys.scheme) or COL_PRIME mode, length L
 template <typename T>
 void launch_col_fused(const ColArgs<T>& a, int L, int ncols, int members, int variant,
-                      hipStream_t st);
+                      hipStream_t st, bool errctl = false);
+// workgroups of the default (variant 0) column launch over ncols columns
+template <typename T>
+int col_fused_blocks(int L, int ncols);
 // 1 if `variant` exists for length L and fits `maxlds` bytes of LDS
 template <typename T>
 int stage_variant_valid(int L, int variant, int maxlds);

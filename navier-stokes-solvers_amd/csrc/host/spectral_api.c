@@ -56,6 +56,14 @@ static void build_params(const NSSpectralData *d, NsdParams *p) {
         p->force_rate = d->device->force_rate;
         p->force_seed = (unsigned long long)f->seed;
     }
+    if (d->device->errctl) {
+        p->errctl = 1;
+        p->err_rtol = d->device->err_rtol;
+        p->err_atol = d->device->err_atol;
+        p->err_safety = d->device->err_safety;
+        p->dt_min = d->device->dt_min;
+        p->dt_max = d->device->dt_max;
+    }
 }
 
 static int sync_params(NSSpectralData *d) {
@@ -527,6 +535,42 @@ int ns_spectral_sync_host(NSSpectralData *d) {
         rc = nsd_sfd_download(d->device->ctx, (double *)d->sfd_hat);
     if (rc == 0 && st != 0) rc = st;
     return set_status(d, rc);
+}
+
+int ns_spectral_set_error_control(NSSpectralData *d, const AdaptiveTimeParams *params,
+                                  double rtol, double atol) {
+    if (!ready(d)) return SOLVER_INVALID_PARAM;
+    if (!params || !params->adapt_enabled) {
+        d->device->errctl = 0;
+        return set_status(d, sync_params(d));
+    }
+    if (!(rtol >= 0.0) || !(atol >= 0.0) || !(rtol > 0.0 || atol > 0.0) ||
+        !(params->safety_factor > 0.0 && params->safety_factor <= 1.0) ||
+        !(params->dt_min > 0.0) || !(params->dt_max >= params->dt_min) ||
+        !(params->cfl_factor > 0.0)) {
+        snprintf(d->device->err, sizeof(d->device->err),
+                 "error control needs rtol, atol >= 0 (not both 0), safety in (0, 1], "
+                 "0 < dt_min <= dt_max and cfl_factor > 0");
+        d->device->status = SOLVER_INVALID_PARAM;
+        return SOLVER_INVALID_PARAM;
+    }
+    d->cfl_factor = params->cfl_factor;
+    d->device->errctl = 1;
+    d->device->err_rtol = rtol;
+    d->device->err_atol = atol;
+    d->device->err_safety = params->safety_factor;
+    d->device->dt_min = params->dt_min;
+    d->device->dt_max = params->dt_max;
+    int rc = sync_params(d);
+    if (rc) d->device->errctl = 0;
+    return set_status(d, rc);
+}
+
+int ns_spectral_error_control_stats(NSSpectralData *d, unsigned long long *accepted,
+                                    unsigned long long *rejected, double *last_err,
+                                    double *dt_next) {
+    if (!ready(d)) return SOLVER_INVALID_PARAM;
+    return set_status(d, nsd_errctl_stats(d->device->ctx, 0, accepted, rejected, last_err, dt_next));
 }
 
 int ns_spectral_device_diagnostics(NSSpectralData *d, double *energy, double *enstrophy) {

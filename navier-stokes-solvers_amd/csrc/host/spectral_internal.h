@@ -7,6 +7,7 @@
 
 #include "ns_device.h"
 #include "ns_spectral_solver.h"
+#include "time_integration.h"
 
 struct NSSpectralDevice {
     NsdContext *ctx;
@@ -16,6 +17,9 @@ struct NSSpectralDevice {
     int force_on;
     NSForcingConfig force_cfg;
     double force_rate;
+    /* adaptive-dt error controller (ns_spectral_set_error_control) */
+    int errctl;
+    double err_rtol, err_atol, err_safety, dt_min, dt_max;
     SolverStatus status;
     char err[256];
     NsdParams applied;

@@ -856,10 +856,46 @@ static void oc_stage_nonlinear(OcData *d) {
  *   d = E a + h E2 k3;   k4 = N(d)
  *   w_{n+1} = E a + h/6 (E k1 + 2 E2 (k2 + k3) + k4)
  * then the white-in-time forcing update w += sqrt(h) f (if f != NULL). */
-void oc_advance_if_rk4(OcData *d, const oc_complex *f) {
+/* Half-plane Parseval weight of column i: 1 on kx = 0 and kx = nx/2, else 2. */
+static double oc_plane_weight(const OcData *d, int idx) {
+    const int i = idx % d->nkx;
+    return (i == 0 || 2 * i == d->nx) ? 1.0 : 2.0;
+}
+
+/* Embedded error estimate of the build-defined adaptive-dt controller: the
+ * step's second stage gives a midpoint solution y2 (order 2) beside the RK4
+ * solution y4; est = y4 - y2 = O(dt^3).  norms[0] = sum w |est|^2 and
+ * norms[1] = sum w |y4|^2 over the stored modes (w: oc_plane_weight), y4
+ * before the forcing kick.  ymid holds y2. */
+static void oc_est_norms(const OcData *d, const oc_complex *y4, const oc_complex *ymid,
+                         double norms[2]) {
+    const int H = d->nkx * d->nky;
+    double e = 0.0, z = 0.0;
+    for (int k = 0; k < H; ++k) {
+        const double w = oc_plane_weight(d, k);
+        const oc_complex est = y4[k] - ymid[k];
+        e += w * (creal(est) * creal(est) + cimag(est) * cimag(est));
+        z += w * (creal(y4[k]) * creal(y4[k]) + cimag(y4[k]) * cimag(y4[k]));
+    }
+    norms[0] = e;
+    norms[1] = z;
+}
+
+static void oc_if_rk4_impl(OcData *d, const oc_complex *f, double *norms);
+
+void oc_advance_if_rk4(OcData *d, const oc_complex *f) { oc_if_rk4_impl(d, f, NULL); }
+
+/* oc_advance_if_rk4 plus the embedded estimate: y2 is the Lawson midpoint
+ * solution E a + h E2 N(b), from the step's own second stage */
+void oc_advance_if_rk4_est(OcData *d, const oc_complex *f, double norms[2]) {
+    oc_if_rk4_impl(d, f, norms);
+}
+
+static void oc_if_rk4_impl(OcData *d, const oc_complex *f, double *norms) {
     const int H = d->nkx * d->nky;
     const double h = d->dt;
     oc_complex *a = d->omega_tmp_hat, *acc = d->rk1_hat, *w = d->omega_hat;
+    oc_complex *ymid = d->rk2_hat;
     memcpy(a, w, (size_t)H * sizeof(oc_complex));
 
     oc_stage_nonlinear(d); /* k1 at a */
@@ -879,6 +915,7 @@ void oc_advance_if_rk4(OcData *d, const oc_complex *f) {
         const oc_complex k2 = d->nonlinear_hat[k];
         acc[k] += (h / 3.0) * E2 * k2;
         w[k] = E2 * a[k] + 0.5 * h * k2;
+        if (norms) ymid[k] = exp(L * h) * a[k] + h * E2 * k2;
     }
     oc_stage_nonlinear(d); /* k3 at c */
 #pragma omp parallel for
@@ -892,10 +929,11 @@ void oc_advance_if_rk4(OcData *d, const oc_complex *f) {
     oc_stage_nonlinear(d); /* k4 at d */
     const double sq = sqrt(h);
 #pragma omp parallel for
-    for (int k = 0; k < H; ++k) {
-        w[k] = acc[k] + (h / 6.0) * d->nonlinear_hat[k];
-        if (f) w[k] += sq * f[k];
-    }
+    for (int k = 0; k < H; ++k) w[k] = acc[k] + (h / 6.0) * d->nonlinear_hat[k];
+    if (norms) oc_est_norms(d, w, ymid, norms);
+    if (f)
+#pragma omp parallel for
+        for (int k = 0; k < H; ++k) w[k] += sq * f[k];
     d->t += h;
     oc_compute_derivatives(d);
 }
@@ -904,7 +942,17 @@ void oc_advance_if_rk4(OcData *d, const oc_complex *f) {
  * algebra (spectral_time_integration.c:44-100) with k_s = N - (k^2/Re + mu) w
  * - HV(w), SFD post-step (:102), then w += sqrt(dt) f.  With mu = 0 and
  * f = NULL this is oc_advance_rk4_minimal. */
-void oc_advance_classic_ext(OcData *d, const oc_complex *f) {
+static void oc_classic_ext_impl(OcData *d, const oc_complex *f, double *norms);
+
+void oc_advance_classic_ext(OcData *d, const oc_complex *f) { oc_classic_ext_impl(d, f, NULL); }
+
+/* oc_advance_classic_ext plus the embedded estimate: y2 = w0 + dt k2, the
+ * explicit midpoint solution from the step's own second stage */
+void oc_advance_classic_ext_est(OcData *d, const oc_complex *f, double norms[2]) {
+    oc_classic_ext_impl(d, f, norms);
+}
+
+static void oc_classic_ext_impl(OcData *d, const oc_complex *f, double *norms) {
     const int H = d->nkx * d->nky;
     const double dt = d->dt;
     oc_complex *ks[4] = {d->rk1_hat, d->rk2_hat, d->rk3_hat, d->rk4_hat};
@@ -924,6 +972,13 @@ void oc_advance_classic_ext(OcData *d, const oc_complex *f) {
 #pragma omp parallel for
     for (int k = 0; k < H; ++k) {
         w[k] = w0[k] + dt / 6.0 * (ks[0][k] + 2.0 * ks[1][k] + 2.0 * ks[2][k] + ks[3][k]);
+    }
+    if (norms) {
+        /* the midpoint solution overwrites k1, which is no longer needed */
+        oc_complex *ymid = ks[0];
+#pragma omp parallel for
+        for (int k = 0; k < H; ++k) ymid[k] = w0[k] + dt * ks[1][k];
+        oc_est_norms(d, w, ymid, norms);
     }
     oc_hyperviscosity_post_step(d, w, dt);
     if (f)

@@ -139,5 +139,9 @@ void oc_forcing_project_and_scale(OcData *d, oc_complex *f, double injection_rat
 void oc_advance_if_rk4(OcData *d, const oc_complex *forcing_scaled);
 void oc_advance_classic_ext(OcData *d, const oc_complex *forcing_scaled);
 void oc_initialize_random_phase(OcData *d, uint64_t seed, double k_peak, double energy);
+/* the same two steps returning the embedded error estimate of the
+ * adaptive-dt controller: norms = {sum w |y4 - y2|^2, sum w |y4|^2} */
+void oc_advance_if_rk4_est(OcData *d, const oc_complex *forcing_scaled, double norms[2]);
+void oc_advance_classic_ext_est(OcData *d, const oc_complex *forcing_scaled, double norms[2]);
 
 #endif

@@ -103,6 +103,8 @@ def lib():
         L.oc_advance_if_rk4.argtypes = [P, c_double_p]
         L.oc_advance_classic_ext.argtypes = [P, c_double_p]
         L.oc_initialize_random_phase.argtypes = [P, C.c_uint64, C.c_double, C.c_double]
+        L.oc_advance_if_rk4_est.argtypes = [P, c_double_p, c_double_p]
+        L.oc_advance_classic_ext_est.argtypes = [P, c_double_p, c_double_p]
         _lib = L
     return _lib
 
@@ -194,6 +196,15 @@ class Oracle:
 
     def advance_classic_ext(self, forcing=None):
         self.L.oc_advance_classic_ext(self.p, None if forcing is None else _dp(forcing))
+
+    def advance_est(self, integrating_factor, forcing=None):
+        """One build-defined step (IF-RK4 or classic with drag) returning the
+        embedded estimate's norms {sum w |y4 - y2|^2, sum w |y4|^2}."""
+        norms = np.zeros(2)
+        fn = self.L.oc_advance_if_rk4_est if integrating_factor else \
+            self.L.oc_advance_classic_ext_est
+        fn(self.p, None if forcing is None else _dp(forcing), _dp(norms))
+        return norms
 
     def cfl(self):
         return self.L.oc_compute_cfl_timestep(self.p)
@@ -310,3 +321,81 @@ def ref_forcing_lib():
     L.ns_forcing_generate.argtypes = [C.POINTER(OcForcingState), C.c_double, c_double_p,
                                       c_double_p]
     return L
+
+
+FAC_MIN, FAC_MAX = 0.2, 5.0
+
+
+class OracleErrorControl:
+    """Build-defined adaptive-dt error controller, restated step for step from
+    the device's sequence.  The reference declares its parameters
+    (AdaptiveTimeParams, include/time_integration.h:16-22: cfl_factor,
+    dt_min, dt_max, safety_factor, adapt_enabled) and never uses them.
+
+    Every attempt first judges the previous candidate step (if one is
+    pending): err = |y4 - y2| / (atol + rtol |y4|) in the half-plane
+    Parseval norm, fac = clamp(safety err^(-1/3), 0.2, 5) (fac = 5 for
+    err = 0).  err <= 1 (or dt at dt_min) accepts; otherwise t goes back by
+    dt and this attempt restores the previous state instead of stepping (a
+    rejected candidate's forcing draw is discarded).  Either way the next
+    step may use dt_err = dt fac.  A stepping attempt takes
+    dt = clamp(min(CFL dt, dt_err), dt_min, dt_max), draws the forcing (if
+    any) for that dt, keeps the state and steps.  finish() judges the last
+    candidate, so that the state left behind is always an accepted one."""
+
+    def __init__(self, o, rtol, atol=0.0, safety=0.9, dt_min=1e-8, dt_max=0.1,
+                 integrating_factor=True, forcing=None, rate=0.0):
+        self.o, self.rtol, self.atol, self.safety = o, rtol, atol, safety
+        self.dt_min, self.dt_max = dt_min, dt_max
+        self.integrating_factor = integrating_factor
+        self.forcing, self.rate = forcing, rate
+        self.pending = self.redo = False
+        self.dt_err = dt_max
+        self.accepted = self.rejected = 0
+        self.err = 0.0
+        self.norms = None
+        self.prev = None
+        self.log = []
+
+    def _judge(self):
+        if not self.pending:
+            self.redo = False
+            return
+        num = float(np.sqrt(self.norms[0]))
+        den = self.atol + self.rtol * float(np.sqrt(self.norms[1]))
+        err = num / den if den > 0 else (float("inf") if num > 0 else 0.0)
+        fac = FAC_MAX if err == 0 else min(FAC_MAX, max(FAC_MIN, self.safety * err ** (-1.0 / 3.0)))
+        dt = self.o.d.dt
+        self.err = err
+        self.pending = False
+        if err <= 1.0 or dt <= self.dt_min:
+            self.accepted += 1
+            self.redo = False
+        else:
+            self.rejected += 1
+            self.o.d.t -= dt
+            self.redo = True
+        self.dt_err = dt * fac
+        self.log.append((dt, err, not self.redo))
+
+    def attempt(self):
+        self._judge()
+        o = self.o
+        if self.redo:
+            o.spec("omega_hat")[:] = self.prev
+            o.compute_derivatives()
+            return
+        o.d.dt = min(max(min(o.cfl(), self.dt_err), self.dt_min), self.dt_max)
+        self.prev = o.spec("omega_hat").copy()
+        f = None
+        if self.forcing is not None:
+            f = o.project_forcing(self.forcing.generate(o.d.dt), self.rate)
+        self.norms = o.advance_est(self.integrating_factor, f)
+        self.pending = True
+
+    def finish(self):
+        self._judge()
+        if self.redo:
+            self.o.spec("omega_hat")[:] = self.prev
+            self.o.compute_derivatives()
+        self.redo = False
