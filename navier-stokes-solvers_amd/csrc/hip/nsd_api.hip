@@ -786,12 +786,13 @@ int lastk(NsdContext* c) {
 }
 
 // The stage kernels' mixed fields use the pair layout (pair_layout in
-// nsd_kernels.hpp) on square FP32 4096^2 grids.  Length 4096 has only the
-// default launch variant (var_ok in nsd_stage.hip), and the kernels
-// static_assert that every instantiation pair_layout selects holds the pair
-// paths, so the layout cannot change with nsd_set_variants.
+// nsd_kernels.hpp) when the row and the column instantiations this context
+// launches both hold it (square FP32 4096^2 grids with their paired
+// transforms); the answer follows the launch variants.
 bool pairs_ok(const NsdContext* c) {
-    return c->prec == NSD_FP32 && c->G.nx == 4096 && c->G.ny == 4096;
+    return c->prec == NSD_FP32 && c->G.nx == c->G.ny &&
+           stage_variant_pairs<float>(c->G.nx, c->vrow) &&
+           stage_variant_pairs<float>(c->G.ny, c->vcol);
 }
 
 template <typename T>
@@ -2069,9 +2070,13 @@ int nsd_set_variants(NsdContext* c, int row_variant, int col_variant) {
         c->err = "launch variant not available for this size";
         return ST_PARAM;
     }
+    const bool pairs_before = pairs_ok(c);
     c->vrow = row_variant;
     c->vcol = col_variant;
     c->gen++;
+    // the mixed fields' layout follows the variants: rewrite them from the
+    // state when it changes (no length has variants of both layouts today)
+    if (pairs_ok(c) != pairs_before) return TYPED(op_prime, c);
     return ST_OK;
 }
 

@@ -443,14 +443,17 @@ __device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
 // one 64-byte line, where the per-field layout puts a field's row pair in a
 // line of its own: half the L1 misses of the row kernel's gathers, whose
 // address path otherwise waits on the cache for about half of its cycles.
-// k_col still writes whole lines (one 16-byte store per lane and pair).  The
-// host selects it per launch (ColArgs/RowArgs::pairs) for square FP32 4096^2
-// grids with the default launch shapes; the transform modes keep the
-// per-field layout.  At 64..2048 it measured slower (256-member ensemble
-// 92k -> 86k member-steps/s), so it stays at 4096.
-template <typename T, int LC>
+// k_col still writes whole lines (one 16-byte store per lane and pair).
+// It is a property of the instantiation: the paired transforms of square
+// FP32 4096^2 grids (the gather-ahead k_row path reads it, the paired k_col
+// path writes it).  The host selects it per launch (ColArgs/RowArgs::pairs)
+// when the row and column instantiations it launches both hold it
+// (stage_variant_pairs); the transform modes keep the per-field layout.  At
+// 64..2048 it measured slower (256-member ensemble 92k -> 86k
+// member-steps/s), so it stays at 4096.
+template <typename T, int LC, int NT, int PR>
 constexpr bool pair_layout() {
-    return sizeof(T) == 4 && LC == 4096;
+    return sizeof(T) == 4 && LC == 4096 && paired_lines<T, LC, NT, PR>();
 }
 
 // store (first, second) at pair element e of pair buffer B (one 16 bytes)
@@ -473,13 +476,12 @@ struct TwSource<true, L, T> {
     using type = TwRegs<L, T>;
 };
 
-template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1, bool ERR = false>
-__global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
-    // a pair-layout length must instantiate the paired writer (the host
-    // selects the layout from the grid alone)
-    static_assert(!(MODE == COL_STAGE || MODE == COL_PRIME) || !pair_layout<T, LC>() ||
-                      paired_lines<T, LC, NT, PR>(),
-                  "pair layout needs the paired k_col path");
+// MW: occupancy floor handed to the register allocator (waves per SIMD)
+template <typename T, int NT, int MODE, int SCHEME, int LC, int PR = -1, bool ERR = false,
+          int MW = min_waves<T>()>
+__global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
+    static_assert(!pair_layout<T, LC, NT, PR>() || paired_lines<T, LC, NT, PR>(),
+                  "the pair layout is written by the paired k_col path");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* smem = reinterpret_cast<cx<T>*>(smem_raw);
     const Geo& G = a.geo;
@@ -792,14 +794,14 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             TwRegs<LC, T> tw;
             load_tw_regs<LC, 0>(tw, j, a.tw);
             if (has_data) {
-                if constexpr (pair_layout<T, LC>()) {
+                if constexpr (pair_layout<T, LC, NT, PR>()) {
                     if (a.pairs) {  // (omega_x, omega_y)
                         cx<T>* pb = a.mixed[2] + 2 * mixed_off;
 #pragma unroll
                         for (int s = 0; s < P; ++s) store_pair(pb, mbase + j + s * TPL, v2[s], v[s]);
                     }
                 }
-                if (!(pair_layout<T, LC>() && a.pairs)) {
+                if (!(pair_layout<T, LC, NT, PR>() && a.pairs)) {
                     cx<T>* o1 = a.mixed[3] + mixed_off;
                     cx<T>* o2 = a.mixed[2] + mixed_off;
 #pragma unroll
@@ -819,14 +821,14 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_col(ColArgs<T> a) {
             }
             line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_y, psi_x
             if (has_data) {
-                if constexpr (pair_layout<T, LC>()) {
+                if constexpr (pair_layout<T, LC, NT, PR>()) {
                     if (a.pairs) {  // (psi_x, psi_y)
                         cx<T>* pb = a.mixed[0] + 2 * mixed_off;
 #pragma unroll
                         for (int s = 0; s < P; ++s) store_pair(pb, mbase + j + s * TPL, v2[s], v[s]);
                     }
                 }
-                if (!(pair_layout<T, LC>() && a.pairs)) {
+                if (!(pair_layout<T, LC, NT, PR>() && a.pairs)) {
                     cx<T>* o1 = a.mixed[1] + mixed_off;
                     cx<T>* o2 = a.mixed[0] + mixed_off;
 #pragma unroll
@@ -1172,11 +1174,12 @@ constexpr bool row_prefetch() {
     return LC >= 64 && (LC != 1024 || sizeof(T) == 4);
 }
 
-template <typename T, int NT, int MODE, int LC, int PR = -1, bool ERR = false>
-__global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
-    static_assert(MODE != ROW_STAGE || !pair_layout<T, LC>() ||
+template <typename T, int NT, int MODE, int LC, int PR = -1, bool ERR = false,
+          int MW = min_waves<T>()>
+__global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
+    static_assert(!pair_layout<T, LC, NT, PR>() ||
                       (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()),
-                  "pair layout needs the paired gather-ahead k_row path");
+                  "the pair layout is read by the paired gather-ahead k_row path");
     // error controller: stages 1-3 of a step that restores a rejected
     // candidate have nothing to do (uniform over the grid)
     if constexpr (ERR && MODE == ROW_STAGE) {
@@ -1274,8 +1277,8 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
         if (blk * lines < G.ny / 2) {
             RowPairRegs<T> r0, r1, q0, q1;
             bool pl = false;
-            if constexpr (pair_layout<T, LC>()) pl = a.pairs != 0;
-            if constexpr (pair_layout<T, LC>()) {
+            if constexpr (pair_layout<T, LC, NT, PR>()) pl = a.pairs != 0;
+            if constexpr (pair_layout<T, LC, NT, PR>()) {
                 if (pl) gather_pair_rows<LC>(r0, r1, j, G, a.mixed[0] + 2 * moff, y0, active);
             }
             if (!pl) {
@@ -1284,7 +1287,7 @@ __global__ __launch_bounds__(NT, min_waves<T>()) void k_row(RowArgs<T> a) {
             }
             stage_two_rows_regs<LC>(lds, j, r0);
             stage_two_rows_regs<LC>(lds2, j, r1);
-            if constexpr (pair_layout<T, LC>()) {
+            if constexpr (pair_layout<T, LC, NT, PR>()) {
                 if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
             }
             if (!pl) {

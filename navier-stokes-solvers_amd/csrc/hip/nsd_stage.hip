@@ -104,6 +104,13 @@ void limits_LV(int bytes) {
 }
 
 template <typename T, int LOG, int V>
+int var_pairs() {
+    constexpr int L = 1 << LOG;
+    if constexpr (var_ok<L, V>()) return pair_layout<T, L, var_nt<L, V>(), var_pr<V>()>() ? 1 : 0;
+    return 0;
+}
+
+template <typename T, int LOG, int V>
 int var_valid(int maxlds) {
     constexpr int L = 1 << LOG;
     if constexpr (var_ok<L, V>()) return var_lds<T, L, V>() <= (size_t)maxlds ? 1 : 0;
@@ -172,6 +179,18 @@ int stage_variant_valid(int L, int variant, int maxlds) {
 }
 
 template <typename T>
+int stage_variant_pairs(int L, int variant) {
+    const int lg = log2i(L);
+#define V_(n, v) \
+    if (lg == n && variant == v) return var_pairs<T, n, v>();
+#define X(n) NSD_FOR_VARS(V_, n)
+    NSD_FOR_LOGS(X)
+#undef X
+#undef V_
+    return 0;
+}
+
+template <typename T>
 void stage_lds_limits(int bytes) {
 #define V_(n, v) limits_LV<T, n, v>(bytes);
 #define X(n) NSD_FOR_VARS(V_, n)
@@ -195,6 +214,7 @@ int col_fused_blocks(int L, int ncols) {
 }
 template int col_fused_blocks<NSD_T>(int, int);
 template int stage_variant_valid<NSD_T>(int, int, int);
+template int stage_variant_pairs<NSD_T>(int, int);
 template void stage_lds_limits<NSD_T>(int);
 
 }  // namespace nsd

@@ -25,6 +25,10 @@ int col_fused_blocks(int L, int ncols);
 // 1 if `variant` exists for length L and fits `maxlds` bytes of LDS
 template <typename T>
 int stage_variant_valid(int L, int variant, int maxlds);
+// 1 if the stage kernels of `variant` at length L read (k_row) and write
+// (k_col) the pair layout of the mixed fields (pair_layout)
+template <typename T>
+int stage_variant_pairs(int L, int variant);
 // raise the dynamic-LDS limit of every instantiation above
 template <typename T>
 void stage_lds_limits(int bytes);
