@@ -252,6 +252,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1, help="GPUs (this build measures one)")
     ap.add_argument("--steps", type=int, default=50, help="timed steps (K)")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--ramp", type=int, default=RAMP_MIN_STEPS,
+                    help="untimed steps before the timed region, warm-up included (clock ramp)")
     ap.add_argument("--n", type=int, default=4096, help="grid size N of the N^2 workload")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (one oracle step, about 20 s)")
@@ -277,7 +279,7 @@ def main():
     ns = G._import_package()
     n = a.n
     r32 = time_gpu(ns, n, True, a.steps, a.warmup, a.probe, compat=0 if a.no_compat else 3,
-                   graph=not a.no_graph, ramp_min=RAMP_MIN_STEPS)
+                   graph=not a.no_graph, ramp_min=a.ramp)
     r64 = None if a.no_fp64 else time_gpu(ns, n, False, max(a.steps // 2, 1), max(a.warmup // 2, 1),
                                           graph=not a.no_graph)
     value = n * n / (r32["ms_per_step"] * 1e-3) / 1e9

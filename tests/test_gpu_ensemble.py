@@ -43,11 +43,11 @@ def single_run(ns, n, Re, seed, steps, fp32, forcing):
     return w, t
 
 
-def make_ensemble(ns, sizes, Re, seeds, fp32, forcing=True):
+def make_ensemble(ns, sizes, Re, seeds, fp32, forcing=True, **cfg):
     return ns.Ensemble(sizes, Re, seeds, forcing=(4.0, 6.0, 1e-3) if forcing else None,
                        precision=ns.NS_PRECISION_FP32 if fp32 else ns.NS_PRECISION_FP64,
                        scheme=ns.NS_SCHEME_IF_RK4, cfl_factor=0.5, drag=0.1, nu4=NU4,
-                       k_peak=10.0, energy=1.0)
+                       k_peak=10.0, energy=1.0, **cfg)
 
 
 @pytest.mark.parametrize("fp32", [False, True])
@@ -165,3 +165,44 @@ def test_launch_variants_give_identical_steps(ns, fp32):
     e, z = s.device_diagnostics()
     assert e == 0.0 and z == 0.0
     s.close()
+
+
+def test_config4_256_members_fp64_deviation(ns):
+    """The 256-member mixed-size ensemble of the benchmark (sizes uniform over
+    {128, 256, 512, 1024}, nu log-uniform in [1e-5, 1e-3], one seed per
+    member, forcing, drag and del^8 as the 4096^2 workload, FP32 stages,
+    autotuned launch shapes) against the same ensemble in FP64: every
+    member's status, its relative L2 deviation after one step (<= 1e-5,
+    FP32 round-off), and its energy and enstrophy after 50 steps (<= 1e-5 and
+    1e-4 relative; the pointwise deviation then grows with each flow's
+    chaotic rate and is printed, not bounded)."""
+    rng = np.random.default_rng(2024)
+    sizes = [int(n) for n in rng.choice([128, 256, 512, 1024], size=256)]
+    nu = np.exp(rng.uniform(np.log(1e-5), np.log(1e-3), size=256))
+    Re = list(1.0 / nu)
+    seeds = list(range(1, 257))
+    e32 = make_ensemble(ns, sizes, Re, seeds, True)
+    e64 = make_ensemble(ns, sizes, Re, seeds, False, autotune=0)
+    for e in (e32, e64):
+        e.advance(1)
+    one = np.array([rel_l2(e32.state(m), e64.state(m)) for m in range(256)])
+    for e in (e32, e64):
+        e.advance(49)
+    t32, E32, Z32, st32 = e32.diagnostics()
+    t64, E64, Z64, st64 = e64.diagnostics()
+    fifty = np.array([rel_l2(e32.state(m), e64.state(m)) for m in range(256)])
+    dE, dZ = np.abs(E32 / E64 - 1), np.abs(Z32 / Z64 - 1)
+    sz = np.array(sizes)
+    for n in (128, 256, 512, 1024):
+        k = sz == n
+        print(f"{n}^2 x {k.sum()}: deviation after 1 step max {one[k].max():.1e} "
+              f"median {np.median(one[k]):.1e}; after 50 steps max {fifty[k].max():.1e}; "
+              f"energy max {dE[k].max():.1e}, enstrophy max {dZ[k].max():.1e}", flush=True)
+    print("per-member FP64 deviation after 1 step: " +
+          " ".join(f"{x:.1e}" for x in one), flush=True)
+    assert np.all(st32 == 0) and np.all(st64 == 0)
+    assert np.all(np.isfinite(one)) and one.max() <= 1e-5
+    assert np.allclose(t32, t64, rtol=1e-5, atol=0)
+    assert dE.max() <= 1e-5 and dZ.max() <= 1e-4
+    for e in (e32, e64):
+        e.close()
