@@ -1,9 +1,10 @@
 /*
  * ns_closure.h -- learned subgrid closure for coarse spectral runs.
  *
- * No reference counterpart: the reference has no closure model.  A 4-layer
- * 3x3 convolutional network (1 -> 32 -> 32 -> 32 -> 1 channels, GELU,
- * periodic padding, 19,105 parameters) maps the coarse vorticity to the
+ * No reference counterpart: the reference has no closure model.  A 3x3
+ * convolutional network of four hidden layers of 32 channels with GELU and a
+ * linear output layer (1 -> 32 -> 32 -> 32 -> 32 -> 1 channels, periodic
+ * padding, 28,353 parameters) maps the coarse vorticity to the
  * subgrid vorticity tendency.  It runs and trains on the GPU with bf16 MFMA
  * and FP32 accumulation; FP32 master parameters.
  *
@@ -53,7 +54,8 @@ int nsd_closure_dataset_slot(NsdClosure *c, int index, float **x, float **t);
 int nsd_closure_dataset_fill(NsdClosure *c, int index, const float *x, const float *t);
 int nsd_closure_dataset_read(NsdClosure *c, int index, float *x, float *t);
 /* Adam (beta 0.9 / 0.999, eps 1e-8) on the mean squared error over 90 % of
- * the `count` pairs in shuffled minibatches; every tenth pair is held out
+ * the `count` pairs in shuffled minibatches, the learning rate decaying from
+ * lr to lr / 20 along a cosine over all steps; every tenth pair is held out
  * for the validation loss.  Losses are in the scaled units (1 = predicting
  * zero). */
 int nsd_closure_train(NsdClosure *c, int count, int epochs, int batch, double lr,

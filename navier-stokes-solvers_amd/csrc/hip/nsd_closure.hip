@@ -1,5 +1,6 @@
-// nsd_closure.hip -- the learned subgrid closure: a 4-layer 3x3 CNN
-// (1 -> 32 -> 32 -> 32 -> 1 channels, GELU, periodic padding) evaluated and
+// nsd_closure.hip -- the learned subgrid closure: a 3x3 CNN of four GELU
+// hidden layers of 32 channels and a linear output layer (1 -> 32 -> 32 ->
+// 32 -> 32 -> 1 channels, periodic padding, 28,353 parameters) evaluated and
 // trained with bf16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation).
 //
 // Layout: activations are NHWC bf16 ([batch][y][x][channel]); weights
@@ -37,7 +38,8 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
 constexpr int kHidden = 32;  // channels of the three hidden layers
-constexpr int kLayers = 4;
+constexpr int kLayers = 5;                  // four GELU hidden layers, linear output
+constexpr int kHiddenLayers = kLayers - 1;
 
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
 // round to nearest even: gfx950's v_cvt_pk_bf16_f32 (NaN stays NaN)
@@ -475,8 +477,8 @@ __global__ void k_adam(int count, float* w, const float* g, float* m, float* v, 
 
 struct NsdClosure {
     int n = 0, lg = 0, npix = 0, max_batch = 0;
-    int cin[kLayers] = {1, kHidden, kHidden, kHidden};
-    int cout[kLayers] = {kHidden, kHidden, kHidden, 1};
+    int cin[kLayers] = {1, kHidden, kHidden, kHidden, kHidden};
+    int cout[kLayers] = {kHidden, kHidden, kHidden, kHidden, 1};
     size_t woff[kLayers] = {}, boff[kLayers] = {}, bwoff[kLayers] = {};
     size_t nparam = 0, nweight = 0;
     float* prm = nullptr;    // FP32 master: per layer W then b
@@ -486,10 +488,10 @@ struct NsdClosure {
     int* widx = nullptr;     // parameter -> bf16 weight index (-1 for biases)
     bf16* wb = nullptr;      // bf16 weights, per layer
     bf16* x0 = nullptr;      // input [B][npix][1]
-    bf16* z[3] = {};         // hidden pre-activations [B][npix][32]
-    bf16* act[3] = {};       // hidden activations
+    bf16* z[kHiddenLayers] = {};    // hidden pre-activations [B][npix][32]
+    bf16* act[kHiddenLayers] = {};  // hidden activations
     float* y = nullptr;      // output [B][npix]
-    bf16* dz[3] = {};        // gradients of the hidden pre-activations
+    bf16* dz[kHiddenLayers] = {};   // gradients of the hidden pre-activations
     bf16* dy = nullptr;      // gradient of the output [B][npix]
     float* xin = nullptr;    // FP32 staging [B][npix]
     float* tin = nullptr;
@@ -522,7 +524,7 @@ unsigned long long splitmix(unsigned long long& s) {
     return z ^ (z >> 31);
 }
 
-// the four layers on `st`, output (times yscale) into y; keep_z keeps the
+// the layers on `st`, output (times yscale) into y; keep_z keeps the
 // pre-activations for a backward pass
 int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStream_t st,
                       bool keep_z) {
@@ -536,14 +538,14 @@ int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStrea
         a.in = l == 0 ? c->x0 : c->act[l - 1];
         a.w = c->wb + c->bwoff[l];
         a.bias = c->prm + c->boff[l];
-        if (l < 3) {
+        if (l < kHiddenLayers) {
             a.z = keep_z ? c->z[l] : nullptr;
             a.act = c->act[l];
         } else {
             a.y = y;
         }
         if (l == 0) launch_conv_fwd<1, kHidden, CONV_FWD_GELU>(a, batch, st);
-        else if (l < 3) launch_conv_fwd<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else if (l < kHiddenLayers) launch_conv_fwd<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
         else launch_conv_fwd<kHidden, 1, CONV_FWD_LAST>(a, batch, st);
     }
     return ck(c, hipGetLastError(), "closure forward") ? 3 : 0;
@@ -580,12 +582,12 @@ int launch_backward(NsdClosure* c, int batch) {
     a.lg = c->lg;
     a.npix = c->npix;
     for (int l = kLayers - 1; l >= 0; --l) {
-        const bf16* dzl = l == 3 ? c->dy : c->dz[l];
+        const bf16* dzl = l == kHiddenLayers ? c->dy : c->dz[l];
         d.x = l == 0 ? c->x0 : c->act[l - 1];
         d.dz = dzl;
         d.dw = c->grad + c->woff[l];
         d.db = c->grad + c->boff[l];
-        if (l == 3)
+        if (l == kHiddenLayers)
             hipLaunchKernelGGL((k_conv_dw<kHidden, 1>), dim3(nblk), dim3(256), 0, c->stream, d);
         else if (l > 0)
             hipLaunchKernelGGL((k_conv_dw<kHidden, kHidden>), dim3(nblk), dim3(256), 0, c->stream, d);
@@ -596,7 +598,7 @@ int launch_backward(NsdClosure* c, int batch) {
         a.w = c->wb + c->bwoff[l];
         a.zprev = c->z[l - 1];
         a.dzprev = c->dz[l - 1];
-        if (l == 3) launch_conv<1, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
+        if (l == kHiddenLayers) launch_conv<1, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
         else launch_conv<kHidden, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
     }
     return ck(c, hipGetLastError(), "closure backward") ? 3 : 0;
@@ -633,12 +635,13 @@ extern "C" {
 void nsd_closure_destroy(NsdClosure* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->prm,    c->grad,   c->adm,    c->adv,  c->widx,  c->wb,   c->x0,
-                    c->z[0],   c->z[1],   c->z[2],   c->act[0], c->act[1], c->act[2], c->y,
-                    c->dz[0],  c->dz[1],  c->dz[2],  c->dy,   c->xin,   c->tin,  c->loss,
-                    c->ds_x,   c->ds_t,   c->idx,    c->sums};
+    void* bufs[] = {c->prm, c->grad, c->adm, c->adv, c->widx, c->wb, c->x0, c->y, c->dy,
+                    c->xin, c->tin, c->loss, c->ds_x, c->ds_t, c->idx, c->sums};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    for (int i = 0; i < kHiddenLayers; ++i)
+        for (void* b : {(void*)c->z[i], (void*)c->act[i], (void*)c->dz[i]})
+            if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -680,7 +683,7 @@ NsdClosure* nsd_closure_create(int n, int max_batch, unsigned long long seed, ch
          al((void**)&c->xin, 4 * P) && al((void**)&c->tin, 4 * P) &&
          al((void**)&c->loss, sizeof(double)) && al((void**)&c->idx, sizeof(int) * max_batch) &&
          al((void**)&c->sums, 2 * sizeof(double));
-    for (int i = 0; i < 3 && ok; ++i)
+    for (int i = 0; i < kHiddenLayers && ok; ++i)
         ok = al((void**)&c->z[i], 2 * P * kHidden) && al((void**)&c->act[i], 2 * P * kHidden) &&
              al((void**)&c->dz[i], 2 * P * kHidden);
     if (!ok) return fail("closure: device allocation failed", c);
@@ -827,6 +830,13 @@ int nsd_closure_train(NsdClosure* c, int count, int epochs, int batch, double lr
     const int ntrain = (int)order.size();
     unsigned long long s = seed ? seed : 0xc105eull;
     const dim3 ggrid((c->npix + 255) / 256 < 64 ? (c->npix + 255) / 256 : 64, batch);
+    // learning rate: cosine decay from lr to lr / 20 over all training steps
+    const long total_steps = (long)epochs * (ntrain / batch);
+    long step = 0;
+    auto rate = [&]() {
+        const double f = total_steps > 1 ? (double)step / (double)(total_steps - 1) : 0.0;
+        return lr * (0.05 + 0.475 * (1.0 + std::cos(3.14159265358979323846 * f)));
+    };
     auto run_batch = [&](const int* ids, int nb, bool learn, double* loss_acc) -> int {
         CKC(hipMemcpyAsync(c->idx, ids, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_gather_batch, dim3(ggrid.x, nb), dim3(256), 0, c->stream, c->npix,
@@ -835,7 +845,8 @@ int nsd_closure_train(NsdClosure* c, int count, int epochs, int batch, double lr
         CKC(hipMemsetAsync(c->loss, 0, sizeof(double), c->stream));
         hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, c->stream, nb * c->npix, c->y, c->tin,
                            c->dy, c->loss);
-        if (learn && (launch_backward(c, nb) || adam_step(c, (float)lr))) return 3;
+        if (learn && (launch_backward(c, nb) || adam_step(c, (float)rate()))) return 3;
+        if (learn) ++step;
         double l = 0.0;
         CKC(hipMemcpyAsync(&l, c->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         CKC(hipStreamSynchronize(c->stream));

@@ -1163,141 +1163,6 @@ __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const Row
     }
 }
 
-// ---- Row stage with the fields of one row packed per transform.
-// Z1 = psi_x + i psi_y and Z2 = omega_y + i omega_x of one physical row give
-// J = psi_x omega_y - psi_y omega_x = Re(Z1 Z2) and u^2 + v^2 = |Z1|^2
-// (spectral_core.c:30-31,101-104): once a row's two transforms are done only
-// its real J row stays live, where the row-packed stage keeps two complex
-// accumulators across the omega transforms.  The J rows of y0 and y0 + 1
-// then share the forward transform as before.
-
-// (F1, F2)(k, y) of this thread's kept slots into (a, b), per-field layout.
-template <int LC, typename T>
-__device__ __forceinline__ void gather_fields_row(RowPairRegs<T>& g, int j, const Geo& G,
-                                                  const cx<T>* F1, const cx<T>* F2, int y,
-                                                  bool active) {
-    constexpr int kxm = Band<LC>::kxm;
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-        const int kind = slot_kind(s, LC, kxm);
-        const int k = j + s * (LC / P);
-        if (kind == 1 || (kind == 3 && k <= kxm)) {
-            if (active) {
-                const unsigned e = col_elem(k, G.ny, y);
-                g.a[s] = *at32(F1, e);
-                g.b[s] = *at32(F2, e);
-            } else {
-                g.a[s] = g.b[s] = mk(T(0), T(0));
-            }
-        }
-    }
-}
-
-// The same from a pair-layout buffer: one 16-byte load per slot returns
-// (first, second) at (k, y); swap puts second into a.
-template <int LC>
-__device__ __forceinline__ void gather_pair_row(RowPairRegs<float>& g, int j, const Geo& G,
-                                                const cx<float>* B, int y, bool active,
-                                                bool swap) {
-    constexpr int kxm = Band<LC>::kxm;
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-        const int kind = slot_kind(s, LC, kxm);
-        const int k = j + s * (LC / P);
-        if (kind == 1 || (kind == 3 && k <= kxm)) {
-            if (active) {
-                const float4 r =
-                    *reinterpret_cast<const float4*>(at32(B, 2 * col_elem(k, G.ny, y)));
-                const cx<float> f = mk(r.x, r.y), sec = mk(r.z, r.w);
-                g.a[s] = swap ? sec : f;
-                g.b[s] = swap ? f : sec;
-            } else {
-                g.a[s] = g.b[s] = mk(0.f, 0.f);
-            }
-        }
-    }
-}
-
-// One row's field pair: from the pair-layout buffer when the instantiation
-// holds that layout (PL) and the launch uses it (pl), else per field.
-template <int LC, bool PL, typename T>
-__device__ __forceinline__ void gather_row_fields(RowPairRegs<T>& g, int j, const Geo& G,
-                                                  const cx<T>* pairbuf, const cx<T>* F1,
-                                                  const cx<T>* F2, int y, bool active, bool pl,
-                                                  bool swap) {
-    if constexpr (PL) {
-        if (pl) {
-            gather_pair_row<LC>(g, j, G, pairbuf, y, active, swap);
-            return;
-        }
-    }
-    gather_fields_row<LC>(g, j, G, F1, F2, y, active);
-}
-
-// sources of one row's field pairs
-template <typename T>
-struct FieldRows {
-    const cx<T>* pb_psi;
-    const cx<T>* pb_om;
-    const cx<T>* psx;
-    const cx<T>* psy;
-    const cx<T>* omx;
-    const cx<T>* omy;
-    bool pl;
-};
-
-// J of one physical row from its gathered field pairs g1 = (psi_x, psi_y),
-// g2 = (omega_y, omega_x): both transforms (in lock-step on two line regions
-// when the instantiation has them), the speed maximum, Re(Z1 Z2).  NEXT:
-// the gathers of row y_next are issued into g1, g2 once their contents are
-// staged.
-template <int LC, int NT, int PR, bool PL, bool NEXT, typename T>
-__device__ __forceinline__ void field_row_J(T (&J)[P], RowPairRegs<T>& g1, RowPairRegs<T>& g2,
-                                            const FieldRows<T>& fr, cx<T>* lds, cx<T>* lds2,
-                                            int j, const Geo& G, const TwRegs<LC, T>& tw,
-                                            int y_next, bool active, int track,
-                                            SpeedMax<T>& um) {
-    cx<T> z1[P], z2[P];
-    if constexpr (line_regions<T, LC, NT, PR>() == 2) {
-        __syncthreads();
-        stage_two_rows_regs<LC>(lds, j, g1);
-        stage_two_rows_regs<LC>(lds2, j, g2);
-        if constexpr (NEXT) {
-            gather_row_fields<LC, PL>(g1, j, G, fr.pb_psi, fr.psx, fr.psy, y_next, active, fr.pl,
-                                      false);
-            gather_row_fields<LC, PL>(g2, j, G, fr.pb_om, fr.omy, fr.omx, y_next, active, fr.pl,
-                                      true);
-        }
-        __syncthreads();
-        lds_read_c<LC>(z1, lds, j);
-        lds_read_c<LC>(z2, lds2, j);
-        line_fft2_c<+1, LC>(z1, z2, lds, lds2, j, tw);  // Z1, Z2
-    } else {
-        __syncthreads();
-        stage_two_rows_regs<LC>(lds, j, g1);
-        if constexpr (NEXT)
-            gather_row_fields<LC, PL>(g1, j, G, fr.pb_psi, fr.psx, fr.psy, y_next, active, fr.pl,
-                                      false);
-        __syncthreads();
-        lds_read_c<LC>(z1, lds, j);
-        line_fft_c<+1, LC>(z1, lds, j, tw);  // Z1
-        __builtin_amdgcn_sched_barrier(0);
-        __syncthreads();
-        stage_two_rows_regs<LC>(lds, j, g2);
-        if constexpr (NEXT)
-            gather_row_fields<LC, PL>(g2, j, G, fr.pb_om, fr.omy, fr.omx, y_next, active, fr.pl,
-                                      true);
-        __syncthreads();
-        lds_read_c<LC>(z2, lds, j);
-        line_fft_c<+1, LC>(z2, lds, j, tw);  // Z2
-    }
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-        if (track) um.add(z1[s].x * z1[s].x + z1[s].y * z1[s].y);
-        J[s] = z1[s].x * z2[s].x - z1[s].y * z2[s].y;
-    }
-}
-
 // Paired row stages that issue the omega gathers ahead of the psi
 // transforms: every compile-time length at FP32, and at FP64 all but 1024,
 // whose closing radix-2 pass (four butterflies per thread) pushes that
@@ -1309,10 +1174,8 @@ constexpr bool row_prefetch() {
     return LC >= 64 && (LC != 1024 || sizeof(T) == 4);
 }
 
-// RV: row-stage formulation, 0 rows packed per transform (f(y0) + i f(y0+1)
-// per field), 1 fields packed per transform (Z1, Z2 of one row, above)
 template <typename T, int NT, int MODE, int LC, int PR = -1, bool ERR = false,
-          int MW = min_waves<T>(), int RV = 0>
+          int MW = min_waves<T>()>
 __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     static_assert(!pair_layout<T, LC, NT, PR>() ||
                       (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()),
@@ -1393,47 +1256,6 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             else v[s] = mk(T(a.in_ref[(size_t)y0 * G.nx + x]), T(a.in_ref[(size_t)(y0 + 1) * G.nx + x]));
         }
         fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
-        return;
-    }
-
-    if constexpr (RV == 1 && LC >= 64) {
-        // fields packed per transform: Z1, Z2 of row y0, then of row y0 + 1;
-        // the next row's gathers are issued as soon as the registers that
-        // held the current row's are staged, and arrive under the transforms
-        // (twiddles in registers, so no global load of the transforms waits
-        // behind them)
-        TwRegs<LC, T> tw;
-        load_tw_regs<LC, 0>(tw, j, a.tw);
-        SpeedMax<T> umf;
-        // (always true: the grid holds no idle workgroup; as a branch it keeps
-        // the schedule of the loads above as compact as the other paths)
-        if (blk * lines < G.ny / 2) {
-            bool pl = false;
-            if constexpr (pair_layout<T, LC, NT, PR>()) pl = a.pairs != 0;
-            constexpr bool PL = pair_layout<T, LC, NT, PR>();
-            FieldRows<T> fr;
-            fr.pb_psi = a.mixed[0] + 2 * moff;  // (psi_x, psi_y) pairs
-            fr.pb_om = a.mixed[2] + 2 * moff;   // (omega_x, omega_y) pairs
-            fr.psx = a.mixed[0] + moff;
-            fr.psy = a.mixed[1] + moff;
-            fr.omx = a.mixed[2] + moff;
-            fr.omy = a.mixed[3] + moff;
-            fr.pl = pl;
-            RowPairRegs<T> g1, g2;
-            T J0[P];
-            gather_row_fields<LC, PL>(g1, j, G, fr.pb_psi, fr.psx, fr.psy, y0, active, pl, false);
-            gather_row_fields<LC, PL>(g2, j, G, fr.pb_om, fr.omy, fr.omx, y0, active, pl, true);
-            field_row_J<LC, NT, PR, PL, true>(J0, g1, g2, fr, lds, lds2, j, G, tw, y0 + 1, active,
-                                              a.track_umax, umf);
-            T J1[P];
-            field_row_J<LC, NT, PR, PL, false>(J1, g1, g2, fr, lds, lds2, j, G, tw, 0, active,
-                                               a.track_umax, umf);
-#pragma unroll
-            for (int s = 0; s < P; ++s) v[s] = mk(J0[s], J1[s]);
-            fwd_rows_store<LC>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
-        }
-        if (a.track_umax)
-            block_max_to(active ? double(umf.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
 

@@ -1,8 +1,9 @@
 """CPU oracle of the subgrid-closure network -- test infrastructure only.
 
-FP64 numpy restatement of the 4-layer 3x3 periodic convolutional network of
-include/ns_closure.h (1 -> 32 -> 32 -> 32 -> 1 channels, GELU in its tanh
-form after the three hidden layers) and of its mean-squared-error gradient.
+FP64 numpy restatement of the 3x3 periodic convolutional network of
+include/ns_closure.h (1 -> 32 -> 32 -> 32 -> 32 -> 1 channels: four hidden
+layers with GELU in its tanh form, then a linear output layer) and of its
+mean-squared-error gradient.
 There is no reference counterpart (the reference has no closure); the
 gradient is pinned by central finite differences and the convolution by a
 direct sum in tests/test_oracle_closure.py.
@@ -13,8 +14,9 @@ for the offset (ky - 1, kx - 1), then the biases [cout].
 import numpy as np
 
 HIDDEN = 32
-CIN = [1, HIDDEN, HIDDEN, HIDDEN]
-COUT = [HIDDEN, HIDDEN, HIDDEN, 1]
+CIN = [1, HIDDEN, HIDDEN, HIDDEN, HIDDEN]
+COUT = [HIDDEN, HIDDEN, HIDDEN, HIDDEN, 1]
+LAYERS = len(CIN)
 C_GELU = np.sqrt(2.0 / np.pi)
 
 
@@ -62,7 +64,7 @@ def forward(params, x):
     for l, (w, b) in enumerate(unpack(np.asarray(params, dtype=np.float64))):
         z = conv(a, w, b)
         cache.append((a, z))
-        a = gelu(z) if l < 3 else z
+        a = gelu(z) if l < LAYERS - 1 else z
     return a[..., 0], cache
 
 
@@ -74,7 +76,7 @@ def loss_and_grad(params, x, t):
     layers = unpack(params)
     grads = []
     dz = (2.0 * d / d.size)[..., None]
-    for l in range(3, -1, -1):
+    for l in range(LAYERS - 1, -1, -1):
         a_in, _ = cache[l]
         w, _ = layers[l]
         dw = np.zeros_like(w)

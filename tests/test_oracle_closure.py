@@ -7,7 +7,8 @@ import closure_oracle as CO
 
 
 def test_param_count():
-    assert CO.param_count() == 19105
+    # 1 -> 32: 9*32 + 32; three 32 -> 32: 3 (9*32*32 + 32); 32 -> 1: 9*32 + 1
+    assert CO.param_count() == 320 + 3 * 9248 + 289 == 28353
 
 
 def test_convolution_matches_direct_sum():
