@@ -225,6 +225,10 @@ def evaluate(net, nf, seeds, t_end, phys=PHYSICS, max_drift=0.10):
         rel = np.abs(E - Ef) / np.maximum(Ef, 1e-300)
         res[name] = {"spectrum_mean_rel_err": float(np.mean(rel)),
                      "spectrum_max_rel_err": float(np.max(rel)),
+                     # shell of the largest error, and the fine spectrum there
+                     # relative to its peak
+                     "spectrum_max_shell": int(np.argmax(rel)) + 1,
+                     "spectrum_max_shell_level": float(Ef[int(np.argmax(rel))] / np.max(Ef)),
                      "enstrophy_rel_err": abs(np.mean([_enstrophy(w) for w in ws]) / Zf - 1.0),
                      "wall_s": wall, "speedup_vs_fine": runs["fine"][1] / wall, "status": st}
     res["closure_accepted"] = res["closure"]["enstrophy_rel_err"] <= max_drift and \
