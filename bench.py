@@ -176,7 +176,8 @@ def time_ensemble(ns, steps, warmup, members=256):
             "all_status_ok": bool(np.all(st == 0))}
 
 
-def closure_workload(ns, seeds=64, snapshots=500, stride=10, epochs=10, members=16, t_end=0.5):
+def closure_workload(ns, seeds=64, snapshots=500, stride=10, epochs=10, members=16, t_end=0.5,
+                     train_only=False):
     """Learned closure: fine 2048^2 forced runs (one per seed) sampled every
     `stride` steps into (coarse vorticity, subgrid tendency) pairs at 256^2
     (64 x 500 = 32,000 pairs, 16.8 GB in HBM), Adam training of the bf16 MFMA
@@ -197,6 +198,12 @@ def closure_workload(ns, seeds=64, snapshots=500, stride=10, epochs=10, members=
     l0, l1, lv = net.train(epochs, batch=16, lr=2e-3)
     t_train = time.perf_counter() - t0
     print(f"closure: {epochs} epochs in {t_train:.1f} s", file=sys.stderr, flush=True)
+    if train_only:  # counter runs of the network kernels: no evaluation
+        net.close()
+        return {"workload": f"2048^2 -> 256^2, {seeds} seeds x {snapshots} snapshots every "
+                            f"{stride} steps, {epochs} epochs; training only",
+                "loss": [round(l0, 4), round(l1, 4)], "validation_loss": round(lv, 4),
+                "data_s": round(t_data, 1), "train_s": round(t_train, 1)}
     r = cl.evaluate(net, 2048, list(range(1001, 1001 + members)), t_end)
     net.close()
     return {"workload": f"2048^2 -> 256^2, {seeds} seeds x {snapshots} snapshots every "
@@ -272,6 +279,9 @@ def main():
                     help="skip training and evaluating the learned closure (about 150 s)")
     ap.add_argument("--closure-snapshots", type=int, default=500,
                     help="training snapshots per fine run (64 runs) of the closure leg")
+    ap.add_argument("--closure-train-only", action="store_true",
+                    help="closure leg without its evaluation, for counter runs of the network "
+                         "kernels")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-member mixed-size ensemble")
     ap.add_argument("--no-compat", action="store_true",
@@ -363,7 +373,8 @@ def main():
     if not a.no_ensemble:
         out["ensemble"] = time_ensemble(ns, max(a.steps // 2, 5), 3)
     if not a.no_closure:
-        out["closure"] = closure_workload(ns, snapshots=a.closure_snapshots)
+        out["closure"] = closure_workload(ns, snapshots=a.closure_snapshots,
+                                          train_only=a.closure_train_only)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

@@ -141,6 +141,14 @@ int nsd_member_download(NsdContext *c, int member, double *omega_hat);
  * across processes (force != 0 re-times).  It zeroes the state: call it
  * before setting the initial field.  Every variant gives identical results. */
 int nsd_autotune(NsdContext *c, int reps, int force, int *row_variant, int *col_variant);
+/* Staggered start of the 4096-point FP32 row stage: the second resident
+ * workgroup of each compute unit starts about 7 us late, so that the two
+ * workgroups of a CU reach their gathers at different times.  It is on when a
+ * probe launch at context creation shows that the device gives every CU one
+ * workgroup before any gets a second (the dispatch order the delay assumes).
+ * set = 0 turns it off, 1 turns it on if that check passed, -1 only queries;
+ * returns the state (1 on, 0 off), -1 on error.  Results do not depend on it. */
+int nsd_row_stagger(NsdContext *c, int set);
 int nsd_set_variants(NsdContext *c, int row_variant, int col_variant);
 
 /* One captured hipGraph that advances several batch contexts (different grid

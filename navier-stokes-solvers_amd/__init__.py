@@ -72,6 +72,7 @@ EXPORTED = [
     "nsd_multi_create", "nsd_multi_destroy", "nsd_multi_error", "nsd_multi_advance",
     "nsd_multi_synchronize", "nsd_autotune", "nsd_set_variants", "nsd_forcing_ou",
     "nsd_forcing_modes", "nsd_physical_to_spectral_noalloc", "nsd_errctl_stats",
+    "nsd_row_stagger",
     # checkpoint.h
     "ns_checkpoint_write_spectral", "ns_checkpoint_read_spectral", "ns_checkpoint_error",
     # ns_closure.h
@@ -333,6 +334,7 @@ def load():
         "nsd_physical_to_spectral_noalloc": ([C.c_int, C.c_int, V, V, V, C.c_char_p,
                                               C.c_size_t], C.c_int),
         "nsd_errctl_stats": ([V, C.c_int, ULLP, ULLP, D, D], C.c_int),
+        "nsd_row_stagger": ([V, C.c_int], C.c_int),
         "ns_checkpoint_write_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
         "ns_checkpoint_read_spectral": ([C.c_char_p, C.POINTER(NSSpectralData)], C.c_int),
         "ns_checkpoint_error": ([], C.c_char_p),

@@ -684,6 +684,65 @@ unsigned grid_for(size_t n, unsigned bs = 256) {
 
 }  // namespace
 
+// ------------------------------------------------------- dispatch order
+
+namespace {
+
+// The staggered start of the 4096-point FP32 row stage delays workgroups
+// [ncu, 2 ncu) on the assumption that the dispatcher gives every compute unit
+// one workgroup before it gives any a second.  k_dispatch_probe checks that
+// assumption on the device: 2 ncu workgroups, each holding the LDS of one
+// 4096-point FP32 row pair (so that at most two share a CU) and resident for
+// about 30 us, record the compute unit they run on, read from the HW_ID
+// (CU, shader array, shader engine) and XCC_ID hardware registers.
+constexpr int kProbeLds = 2 * (4096 + 4096 / 16) * 8;
+
+__global__ __launch_bounds__(512) void k_dispatch_probe(unsigned* cu_of) {
+    extern __shared__ unsigned char probe_lds[];
+    if (threadIdx.x == 0) {
+        // s_getreg immediates: (size - 1) << 11 | offset << 6 | register
+        const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);   // HW_ID, 32 bits
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20);  // XCC_ID, bits 3:0
+        probe_lds[0] = 1;
+        cu_of[blockIdx.x] = (xcc << 8) | ((hw >> 8) & 0x7fu);  // CU 11:8, SH 12, SE 14:13
+    }
+    for (int i = 0; i < 8; ++i) __builtin_amdgcn_s_sleep(127);
+}
+
+// 1 when workgroups [0, ncu) of a launch land on ncu different compute units
+// and each unit holds exactly two of the first 2 ncu; cached per device
+int dispatch_breadth_first(int dev, int ncu) {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int ok = 0;
+    unsigned* d = nullptr;
+    std::vector<unsigned> h(2 * (size_t)ncu);
+    if (ncu > 0 && hipMalloc(&d, h.size() * sizeof(unsigned)) == hipSuccess) {
+        (void)hipFuncSetAttribute((const void*)k_dispatch_probe,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kProbeLds);
+        hipLaunchKernelGGL(k_dispatch_probe, dim3(2 * ncu), dim3(512), kProbeLds, 0, d);
+        if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+            hipMemcpy(h.data(), d, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost) ==
+                hipSuccess) {
+            std::map<unsigned, int> first, all;
+            for (int b = 0; b < 2 * ncu; ++b) {
+                if (b < ncu) first[h[b]]++;
+                all[h[b]]++;
+            }
+            ok = (int)first.size() == ncu && (int)all.size() == ncu;
+            for (auto& kv : all) ok = ok && kv.second == 2;
+        }
+        (void)hipFree(d);
+    }
+    cache[dev] = ok;
+    return ok;
+}
+
+}  // namespace
+
 // ------------------------------------------------------------------ context
 
 struct NsdContext {
@@ -693,6 +752,7 @@ struct NsdContext {
     int vrow = 0, vcol = 0;                 // launch variants of the step kernels
     int maxlds = 65536;                     // LDS bytes per workgroup on this device
     int ncu = 0;                            // compute units on this device
+    int stagger = 0;                        // staggered start of the 4096-point FP32 row stage
     double* m_inv_re = nullptr;             // per-member 1/Re (batches), device
     unsigned long long* m_seed = nullptr;   // per-member forcing seeds (batches), device
     unsigned long long* m_ic = nullptr;     // per-member initial-field seeds, device
@@ -938,8 +998,8 @@ void enqueue_stage(NsdContext* c, int s, int adaptive) {
     r.mode = ROW_STAGE;
     r.track_umax = (s == 0) ? 1 : 0;
     r.skip_on_redo = (s > 0) ? 1 : 0;
-    r.stagger_lo = c->ncu;  // second resident workgroup of each CU
-    r.stagger_hi = 2 * c->ncu;
+    r.stagger_lo = c->stagger ? c->ncu : 0;  // second resident workgroup of each CU
+    r.stagger_hi = c->stagger ? 2 * c->ncu : 0;
     ColArgs<T> a = col_args<T>(c);
     a.mode = COL_STAGE;
     a.stage = s;
@@ -1673,6 +1733,7 @@ NsdContext* nsd_create_batch(int nx, int ny, double Lx, double Ly, int precision
     if (precision == NSD_FP32) set_lds_limits<float>(maxlds);
     else set_lds_limits<double>(maxlds);
     c->maxlds = maxlds;
+    if (precision == NSD_FP32 && nx == 4096) c->stagger = dispatch_breadth_first(dev, c->ncu);
     nsd_set_params(c, &c->P);
     return c;
 }
@@ -1921,6 +1982,17 @@ int nsd_errctl_stats(NsdContext* c, int member, unsigned long long* accepted,
     if (last_err) *last_err = s.err;
     if (dt_next) *dt_next = s.dt_err;
     return ST_OK;
+}
+
+int nsd_row_stagger(NsdContext* c, int set) {
+    if (!c) return -1;
+    if (set >= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return -1;
+        c->stagger = set && dispatch_breadth_first(dev, c->ncu);
+        c->gen++;  // kernel arguments changed: re-capture the step graph
+    }
+    return c->stagger;
 }
 
 int nsd_advance(NsdContext* c, int nsteps, int adaptive) {

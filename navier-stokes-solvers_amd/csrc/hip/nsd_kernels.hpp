@@ -1200,8 +1200,10 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     // Staggered start.  The dispatcher gives every compute unit one workgroup
-    // before it gives any a second, so blocks [ncu, 2 ncu) of member 0 are the
-    // second residents; they start about 16k cycles late.  The two workgroups
+    // before it gives any a second (the host checks this on the device when a
+    // context is created and passes an empty range otherwise), so blocks
+    // [ncu, 2 ncu) of member 0 are the second residents; they start about 16k
+    // cycles late.  The two workgroups
     // of a CU then reach their gather phases at different times, each loading
     // under the other's transforms, and keep that offset for the whole launch
     // (a freed slot is refilled at once).  At 4096 (FP32, two resident

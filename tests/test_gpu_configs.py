@@ -180,3 +180,25 @@ def test_pair_layout_transitions_4096_fp32(ns):
     assert np.array_equal(f32.spec("omega_hat"), plain.spec("omega_hat"))
     for s in made:
         s.close()
+
+
+def test_row_stagger_dispatch_check_and_identical_steps(ns):
+    """The 4096-point FP32 row stage delays the second resident workgroup of
+    each compute unit.  A probe launch at context creation checks the
+    dispatch order that delay assumes (every CU gets one workgroup before any
+    gets a second); the stagger is on only if it holds, and on the MI355X it
+    must.  With the stagger on and off the step gives the same bits."""
+    runs = [forced_4096(ns, True), forced_4096(ns, True)]
+    on = runs[0].L.nsd_row_stagger(runs[0].ctx, -1)
+    print(f"dispatch check: stagger {'on' if on == 1 else 'off'}", flush=True)
+    assert on == 1, "the device did not give every CU one workgroup before a second"
+    assert runs[1].L.nsd_row_stagger(runs[1].ctx, 0) == 0
+    assert runs[1].L.nsd_row_stagger(runs[1].ctx, -1) == 0
+    for s in runs:
+        assert s.advance_n(3, True) == 0
+        assert s.sync_host() == 0
+    assert runs[0].d.t == runs[1].d.t
+    assert np.array_equal(runs[0].spec("omega_hat"), runs[1].spec("omega_hat"))
+    assert runs[1].L.nsd_row_stagger(runs[1].ctx, 1) == 1
+    for s in runs:
+        s.close()
