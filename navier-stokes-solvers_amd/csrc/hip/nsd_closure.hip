@@ -1,7 +1,7 @@
 // nsd_closure.hip -- the learned subgrid closure: a 3x3 CNN of four GELU
 // hidden layers of 32 channels and a linear output layer (1 -> 32 -> 32 ->
 // 32 -> 32 -> 1 channels, periodic padding, 28,353 parameters) evaluated and
-// trained with bf16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation).
+// trained with bf16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
 //
 // Layout: activations are NHWC bf16 ([batch][y][x][channel]); weights
 // [cout][tap][cin] with tap = 3 ky + kx for the offset (ky - 1, kx - 1);
@@ -15,7 +15,10 @@
 // depth 32: one 3x3 tap of 32 channels); the B operand (at most 288 x 32
 // bf16) sits in LDS, transposed so each lane's 8 k-values are one 16-byte
 // read, and each lane's A values are one 16-byte load of 8 channels.  A workgroup stages B once and walks TILES tiles of 64
-// pixels, so the staging is paid once per 64 x TILES pixels.
+// pixels, so the staging is paid once per 64 x TILES pixels.  Grids that are
+// multiples of 64 use the LDS-tiled kernels instead: k_conv_tile (forward and
+// input gradient) and k_conv_dw_tile / k_conv_dw_tile_in (weight gradients);
+// k_conv and k_conv_dw serve the smaller grids.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -183,7 +186,8 @@ void launch_conv(const ConvArgs& a, int batch, hipStream_t st) {
                            a);
 }
 
-// Forward layers from an LDS-staged input patch.  A workgroup of 512
+// Layers from an LDS-staged input patch (forward, and the input gradient,
+// whose A operand is dZ shifted the other way).  A workgroup of 512
 // threads computes a tile of 8 rows x 64 pixels of one member: it stages the
 // tile's input with its one-pixel periodic halo (10 x 66 pixels, all KC
 // channels) in LDS with coalesced 16-byte loads issued together, holds its
@@ -196,8 +200,8 @@ constexpr int kTileRows = 8, kTileCols = 64;
 
 template <int KC, int NN, int MODE>
 __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
-    static_assert(MODE == CONV_FWD_GELU || MODE == CONV_FWD_LAST, "forward layers only");
     static_assert(kTileRows * 64 == 512, "one wave per tile row");
+    constexpr int S = MODE == CONV_BWD_DATA ? -1 : 1;  // tap offsets applied to the input
     constexpr int KP = (9 * KC + 31) / 32 * 32;
     constexpr int NP = (NN + 15) / 16 * 16;
     constexpr int PR = kTileRows + 2, PC = kTileCols + 2;
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
         bf16 v = 0;
         if (nn < NN && k < 9 * KC) {
             const int tap = k / KC, kc = k % KC;
-            v = a.w[(nn * 9 + tap) * KC + kc];
+            v = MODE == CONV_BWD_DATA ? a.w[(kc * 9 + tap) * NN + nn] : a.w[(nn * 9 + tap) * KC + kc];
         }
         Bt[e] = v;
     }
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
             s8 av;
             if constexpr (KC % 32 == 0) {
                 const int tap = k0 / KC, kc = k0 % KC;
-                const int pr = lr + 1 + (tap / 3 - 1), pc = lc + m + 1 + (tap % 3 - 1);
+                const int pr = lr + 1 + S * (tap / 3 - 1), pc = lc + m + 1 + S * (tap % 3 - 1);
                 av = *reinterpret_cast<const s8*>(&Xs[(pr * PC + pc) * KC + kc]);
             } else {
 #pragma unroll
@@ -269,7 +273,8 @@ __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
                     short v = 0;
                     if (k < 9 * KC) {
                         const int tap = k / KC, kc = k % KC;
-                        const int pr = lr + 1 + (tap / 3 - 1), pc = lc + m + 1 + (tap % 3 - 1);
+                        const int pr = lr + 1 + S * (tap / 3 - 1),
+                                  pc = lc + m + 1 + S * (tap % 3 - 1);
                         v = (short)Xs[(pr * PC + pc) * KC + kc];
                     }
                     av[i] = v;
@@ -298,9 +303,14 @@ __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
                 for (int i = 0; i < 4; ++i) {
                     const size_t q = q0 + 4 * kq + i;
                     if (nn < NN) {
-                        const float zz = acc[nb][i] + a.bias[nn];
-                        if (a.z) a.z[q * NN + nn] = f2bf(zz);
-                        a.act[q * NN + nn] = f2bf(gelu(zz));
+                        if constexpr (MODE == CONV_BWD_DATA) {
+                            a.dzprev[q * NN + nn] =
+                                f2bf(acc[nb][i] * gelu_d(bf2f(a.zprev[q * NN + nn])));
+                        } else {
+                            const float zz = acc[nb][i] + a.bias[nn];
+                            if (a.z) a.z[q * NN + nn] = f2bf(zz);
+                            a.act[q * NN + nn] = f2bf(gelu(zz));
+                        }
                     }
                 }
             }
@@ -308,9 +318,9 @@ __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
     }
 }
 
-// forward layers: the tiled kernel where the grid is a multiple of its tile
+// the tiled kernel where the grid is a multiple of its tile
 template <int KC, int NN, int MODE>
-void launch_conv_fwd(const ConvArgs& a, int batch, hipStream_t st) {
+void launch_conv_lds(const ConvArgs& a, int batch, hipStream_t st) {
     if (a.n % kTileCols == 0)
         hipLaunchKernelGGL((k_conv_tile<KC, NN, MODE>),
                            dim3((a.n / kTileCols) * (a.n / kTileRows), batch), dim3(512), 0, st, a);
@@ -395,6 +405,285 @@ __global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
             const int mrow = mb * 16 + 4 * kq + i;
             if (mrow < 9 * CIN && co < COUT) atomicAdd(&a.dw[co * 9 * CIN + mrow], acc[t][i]);
         }
+    }
+}
+
+// Weight gradient of a 32-channel-input layer from LDS-staged tiles (n a
+// multiple of 64).  dW^T[(tap, ci)][co] = sum_p X[p + off(tap)][ci] dZ[p][co]
+// is a GEMM of depth "pixels".  A workgroup of six waves walks `per_wg` tiles
+// of 4 rows x 64 pixels of one member; per tile it stages
+//   Xl[ci][yy][xx]: the input with its one-pixel periodic halo, channel-major
+//                   (tile row y at yy = y + 1, column x at xx = x + 8), and
+//   Dl[co][y][x]:   dZ, channel-major (zero rows up to 16 channels),
+// so that 8 consecutive pixels of one channel are 16 contiguous bytes.  Wave
+// (ky, h) owns the three taps (ky, kx = 0..2) of channels 16 h .. 16 h + 15:
+// per 32 pixels a lane reads one 16-byte chunk of its row (plus one dword on
+// each side), forms the three column shifts with byte-align operations, and
+// issues 3 x NT MFMAs (v_mfma_f32_16x16x32_bf16).  Partial sums, and the
+// bias gradient, go to part[workgroup][...]; k_dw_reduce adds them up in a
+// fixed order.
+constexpr int kDwRows = 4, kDwCols = 64;
+constexpr int kDwXs = 80;                     // LDS row stride of Xl (elements)
+constexpr int kDwXc = (kDwRows + 2) * kDwXs;  // LDS channel stride of Xl
+constexpr int kDwMaxWg = 1024;                // workgroups of one launch, at most
+
+struct DwTileArgs {
+    int n, lg, npix;
+    int tiles;         // members x (n / 4) x (n / 64)
+    int per_wg;        // tiles per workgroup
+    const bf16* x;     // layer input [member][npix][32]
+    const bf16* dz;    // [member][npix][COUT]
+    float* part;       // [gridDim.x][9 x 32 x COUT + COUT]
+};
+
+__device__ __forceinline__ unsigned align2(unsigned hi, unsigned lo) {
+    return __builtin_amdgcn_alignbyte(hi, lo, 2);
+}
+
+template <int COUT>
+__global__ __launch_bounds__(384) void k_conv_dw_tile(DwTileArgs a) {
+    constexpr int CIN = 32;
+    constexpr int NT = (COUT + 15) / 16;
+    constexpr int DC = NT * 16;
+    constexpr int NW = 9 * CIN * COUT;
+    __shared__ __align__(16) bf16 Xl[CIN * kDwXc];
+    __shared__ __align__(16) bf16 Dl[DC * kDwRows * kDwCols];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ky = wave >> 1, h = wave & 1;
+    const int r = lane & 15, g = lane >> 4;
+    f4 acc[3][NT];
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+        for (int nb = 0; nb < NT; ++nb) acc[kx][nb] = f4{0.f, 0.f, 0.f, 0.f};
+    float dbs = 0.f;  // bias gradient: thread 8 co + seg sums a quarter row of Dl[co]
+    if constexpr (COUT < DC) {
+        for (int e = COUT * kDwRows * kDwCols + tid; e < DC * kDwRows * kDwCols; e += 384)
+            Dl[e] = 0;
+    }
+    const int tx_n = a.n / kDwCols, ty_n = a.n / kDwRows;
+    const int t_begin = blockIdx.x * a.per_wg;
+    const int t_end = min(t_begin + a.per_wg, a.tiles);
+    for (int t = t_begin; t < t_end; ++t) {
+        const int mem = t / (tx_n * ty_n), rem = t - mem * (tx_n * ty_n);
+        const int y0 = (rem / tx_n) * kDwRows, x0 = (rem % tx_n) * kDwCols;
+        const bf16* xb = a.x + (size_t)mem * a.npix * CIN;
+        const bf16* db = a.dz + (size_t)mem * a.npix * COUT;
+        __syncthreads();  // the previous tile's reads are done
+        {   // interior: 6 rows x 16 quads of pixels x 4 groups of 8 channels
+            const int cg = tid & 3, qd = (tid >> 2) & 15, yy = tid >> 6;
+            const int gy = (y0 + yy - 1) & (a.n - 1);
+            const bf16* src = xb + ((size_t)(gy << a.lg) + x0 + 4 * qd) * CIN + 8 * cg;
+            const s8 v0 = *reinterpret_cast<const s8*>(src);
+            const s8 v1 = *reinterpret_cast<const s8*>(src + CIN);
+            const s8 v2 = *reinterpret_cast<const s8*>(src + 2 * CIN);
+            const s8 v3 = *reinterpret_cast<const s8*>(src + 3 * CIN);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const s4 q = {v0[c], v1[c], v2[c], v3[c]};
+                *reinterpret_cast<s4*>(&Xl[(8 * cg + c) * kDwXc + yy * kDwXs + 8 + 4 * qd]) = q;
+            }
+        }
+        {   // halo columns x = -1 and x = 64: 6 rows x 2 sides x 32 channels
+            const int side = tid & 1, ci = (tid >> 1) & 31, yy = tid >> 6;
+            const int gy = (y0 + yy - 1) & (a.n - 1);
+            const int gx = (side ? x0 + kDwCols : x0 - 1) & (a.n - 1);
+            Xl[ci * kDwXc + yy * kDwXs + (side ? 8 + kDwCols : 7)] =
+                xb[((size_t)(gy << a.lg) + gx) * CIN + ci];
+        }
+        if constexpr (COUT == 32) {  // 4 rows x 16 quads x 4 groups of 8 channels
+            if (tid < 256) {
+                const int cg = tid & 3, qd = (tid >> 2) & 15, yr = tid >> 6;
+                const bf16* src =
+                    db + ((size_t)((y0 + yr) << a.lg) + x0 + 4 * qd) * COUT + 8 * cg;
+                const s8 v0 = *reinterpret_cast<const s8*>(src);
+                const s8 v1 = *reinterpret_cast<const s8*>(src + COUT);
+                const s8 v2 = *reinterpret_cast<const s8*>(src + 2 * COUT);
+                const s8 v3 = *reinterpret_cast<const s8*>(src + 3 * COUT);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const s4 q = {v0[c], v1[c], v2[c], v3[c]};
+                    *reinterpret_cast<s4*>(
+                        &Dl[(8 * cg + c) * (kDwRows * kDwCols) + yr * kDwCols + 4 * qd]) = q;
+                }
+            }
+        } else {  // one channel: rows of 64 contiguous values
+            static_assert(COUT == 1, "weight-gradient tiles for 32 or 1 output channels");
+            if (tid < 32) {
+                const int yr = tid >> 3, seg = tid & 7;
+                *reinterpret_cast<s8*>(&Dl[yr * kDwCols + 8 * seg]) = *reinterpret_cast<const s8*>(
+                    db + ((size_t)((y0 + yr) << a.lg) + x0 + 8 * seg));
+            }
+        }
+        __syncthreads();
+        if (tid < 8 * COUT) {  // bias: channel tid / 8, pixels 32 (tid % 8) .. + 31
+            const bf16* d = &Dl[(tid >> 3) * (kDwRows * kDwCols) + 32 * (tid & 7)];
+#pragma unroll
+            for (int e = 0; e < 32; ++e) dbs += bf2f(d[e]);
+        }
+#pragma unroll
+        for (int yr = 0; yr < kDwRows; ++yr) {
+#pragma unroll
+            for (int hx = 0; hx < 2; ++hx) {
+                const int xs = 32 * hx + 8 * g;  // this lane's first pixel in the tile row
+                const bf16* xrow = &Xl[(16 * h + r) * kDwXc + (yr + ky) * kDwXs + xs];
+                const unsigned lo = *reinterpret_cast<const unsigned*>(xrow + 6);
+                const uint4 mid = *reinterpret_cast<const uint4*>(xrow + 8);
+                const unsigned hi = *reinterpret_cast<const unsigned*>(xrow + 16);
+                const unsigned u1 = align2(mid.y, mid.x), u2 = align2(mid.z, mid.y),
+                               u3 = align2(mid.w, mid.z);
+                const uint4 av[3] = {uint4{align2(mid.x, lo), u1, u2, u3}, mid,
+                                     uint4{u1, u2, u3, align2(hi, mid.w)}};
+#pragma unroll
+                for (int nb = 0; nb < NT; ++nb) {
+                    const uint4 bv = *reinterpret_cast<const uint4*>(
+                        &Dl[(16 * nb + r) * (kDwRows * kDwCols) + yr * kDwCols + xs]);
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx)
+                        acc[kx][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, av[kx]), __builtin_bit_cast(bf8, bv),
+                            acc[kx][nb], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D[row = channel 16 h + 4 g + i][col = output channel 16 nb + r] of tap (ky, kx)
+    float* out = a.part + (size_t)blockIdx.x * (NW + COUT);
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+        for (int nb = 0; nb < NT; ++nb) {
+            const int co = 16 * nb + r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (co < COUT) out[(co * 9 + 3 * ky + kx) * CIN + 16 * h + 4 * g + i] = acc[kx][nb][i];
+        }
+    if (tid < 8 * COUT) {  // eight neighbouring lanes hold one channel's parts
+        dbs += __shfl_xor(dbs, 1);
+        dbs += __shfl_xor(dbs, 2);
+        dbs += __shfl_xor(dbs, 4);
+        if ((tid & 7) == 0) out[NW + (tid >> 3)] = dbs;
+    }
+}
+
+// The same for the one-channel input layer: dW^T[tap][co] = sum_p X[p +
+// off(tap)] dZ[p][co], nine taps padded to one 16-row MFMA tile.  Four waves,
+// one per tile row; lane row r < 9 is tap (r / 3, r % 3) and picks its column
+// shift from the three byte-aligned forms.  Each wave writes its own partial
+// row (part[4 workgroup + wave][9 x 32 + 32]).
+template <int COUT>
+__global__ __launch_bounds__(256) void k_conv_dw_tile_in(DwTileArgs a) {
+    static_assert(COUT == 32, "input-layer tiles for 32 output channels");
+    constexpr int NT = COUT / 16;
+    constexpr int NW = 9 * COUT;
+    __shared__ __align__(16) bf16 Xl[kDwXc];
+    __shared__ __align__(16) bf16 Dl[COUT * kDwRows * kDwCols];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int tky = r / 3, tkx = r - 3 * (r / 3);  // tap of this lane's A row (r < 9)
+    f4 acc[NT];
+#pragma unroll
+    for (int nb = 0; nb < NT; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+    float dbs = 0.f;
+    const int tx_n = a.n / kDwCols, ty_n = a.n / kDwRows;
+    const int t_begin = blockIdx.x * a.per_wg;
+    const int t_end = min(t_begin + a.per_wg, a.tiles);
+    for (int t = t_begin; t < t_end; ++t) {
+        const int mem = t / (tx_n * ty_n), rem = t - mem * (tx_n * ty_n);
+        const int y0 = (rem / tx_n) * kDwRows, x0 = (rem % tx_n) * kDwCols;
+        const bf16* xb = a.x + (size_t)mem * a.npix;
+        const bf16* db = a.dz + (size_t)mem * a.npix * COUT;
+        __syncthreads();
+        if (tid < 48) {  // interior: 6 rows x 8 chunks of 8 pixels
+            const int yy = tid >> 3, seg = tid & 7;
+            const int gy = (y0 + yy - 1) & (a.n - 1);
+            *reinterpret_cast<s8*>(&Xl[yy * kDwXs + 8 + 8 * seg]) =
+                *reinterpret_cast<const s8*>(xb + ((size_t)(gy << a.lg) + x0 + 8 * seg));
+        } else if (tid < 60) {  // halo columns: 6 rows x 2 sides
+            const int e = tid - 48, yy = e >> 1, side = e & 1;
+            const int gy = (y0 + yy - 1) & (a.n - 1);
+            const int gx = (side ? x0 + kDwCols : x0 - 1) & (a.n - 1);
+            Xl[yy * kDwXs + (side ? 8 + kDwCols : 7)] = xb[(size_t)(gy << a.lg) + gx];
+        }
+        {   // dZ: 4 rows x 16 quads x 4 groups of 8 channels
+            const int cg = tid & 3, qd = (tid >> 2) & 15, yr = tid >> 6;
+            const bf16* src = db + ((size_t)((y0 + yr) << a.lg) + x0 + 4 * qd) * COUT + 8 * cg;
+            const s8 v0 = *reinterpret_cast<const s8*>(src);
+            const s8 v1 = *reinterpret_cast<const s8*>(src + COUT);
+            const s8 v2 = *reinterpret_cast<const s8*>(src + 2 * COUT);
+            const s8 v3 = *reinterpret_cast<const s8*>(src + 3 * COUT);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const s4 q = {v0[c], v1[c], v2[c], v3[c]};
+                *reinterpret_cast<s4*>(
+                    &Dl[(8 * cg + c) * (kDwRows * kDwCols) + yr * kDwCols + 4 * qd]) = q;
+            }
+        }
+        __syncthreads();
+        {   // bias: channel tid / 8, pixels 32 (tid % 8) .. + 31
+            const bf16* d = &Dl[(tid >> 3) * (kDwRows * kDwCols) + 32 * (tid & 7)];
+#pragma unroll
+            for (int e = 0; e < 32; ++e) dbs += bf2f(d[e]);
+        }
+        const int yr = wave;
+#pragma unroll
+        for (int hx = 0; hx < 2; ++hx) {
+            const int xs = 32 * hx + 8 * g;
+            uint4 av = uint4{0u, 0u, 0u, 0u};
+            if (r < 9) {
+                const bf16* xrow = &Xl[(yr + tky) * kDwXs + xs];
+                const unsigned lo = *reinterpret_cast<const unsigned*>(xrow + 6);
+                const uint4 mid = *reinterpret_cast<const uint4*>(xrow + 8);
+                const unsigned hi = *reinterpret_cast<const unsigned*>(xrow + 16);
+                const unsigned u1 = align2(mid.y, mid.x), u2 = align2(mid.z, mid.y),
+                               u3 = align2(mid.w, mid.z);
+                av = tkx == 1 ? mid
+                     : tkx == 0 ? uint4{align2(mid.x, lo), u1, u2, u3}
+                                : uint4{u1, u2, u3, align2(hi, mid.w)};
+            }
+#pragma unroll
+            for (int nb = 0; nb < NT; ++nb) {
+                const uint4 bv = *reinterpret_cast<const uint4*>(
+                    &Dl[(16 * nb + r) * (kDwRows * kDwCols) + yr * kDwCols + xs]);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv), acc[nb], 0, 0, 0);
+            }
+        }
+    }
+    // D[row = tap 4 g + i][col = output channel 16 nb + r]
+    float* out = a.part + ((size_t)blockIdx.x * 4 + wave) * (NW + COUT);
+#pragma unroll
+    for (int nb = 0; nb < NT; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (4 * g + i < 9) out[(16 * nb + r) * 9 + 4 * g + i] = acc[nb][i];
+    dbs += __shfl_xor(dbs, 1);
+    dbs += __shfl_xor(dbs, 2);
+    dbs += __shfl_xor(dbs, 4);
+    // this wave's bias channels are 8 wave .. 8 wave + 7; the others are zero in its row
+    if ((lane & 7) == 0) out[NW + (tid >> 3)] = dbs;
+    if (lane < COUT && (lane >> 3) != wave) out[NW + lane] = 0.f;
+}
+
+// grad[i] += sum over workgroups w of part[w][i]: 64 outputs per block; wave
+// q sums the workgroups w = q mod 4 in order, and the four sums are added in
+// a fixed order (the result does not depend on timing)
+__global__ __launch_bounds__(256) void k_dw_reduce(int nwg, int stride, const float* part,
+                                                   float* dw, int nw, float* db) {
+    __shared__ float red[4][64];
+    const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + o;
+    float s = 0.f;
+    if (i < stride) {
+#pragma unroll 8
+        for (int w = q; w < nwg; w += 4) s += part[(size_t)w * stride + i];
+    }
+    red[q][o] = s;
+    __syncthreads();
+    if (q == 0 && i < stride) {
+        const float t = (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
+        if (i < nw) dw[i] += t;
+        else db[i - nw] += t;
     }
 }
 
@@ -503,6 +792,8 @@ struct NsdClosure {
     int ds_count = 0;
     int* idx = nullptr;      // minibatch indices
     double* sums = nullptr;
+    float* dwpart = nullptr;  // weight-gradient partial sums per workgroup
+    int dw_wg = 0;            // their rows
     hipStream_t stream = nullptr;
     std::string err;
 };
@@ -544,9 +835,9 @@ int launch_forward_to(NsdClosure* c, int batch, float* y, float yscale, hipStrea
         } else {
             a.y = y;
         }
-        if (l == 0) launch_conv_fwd<1, kHidden, CONV_FWD_GELU>(a, batch, st);
-        else if (l < kHiddenLayers) launch_conv_fwd<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
-        else launch_conv_fwd<kHidden, 1, CONV_FWD_LAST>(a, batch, st);
+        if (l == 0) launch_conv_lds<1, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else if (l < kHiddenLayers) launch_conv_lds<kHidden, kHidden, CONV_FWD_GELU>(a, batch, st);
+        else launch_conv_lds<kHidden, 1, CONV_FWD_LAST>(a, batch, st);
     }
     return ck(c, hipGetLastError(), "closure forward") ? 3 : 0;
 }
@@ -587,7 +878,42 @@ int launch_backward(NsdClosure* c, int batch) {
         d.dz = dzl;
         d.dw = c->grad + c->woff[l];
         d.db = c->grad + c->boff[l];
-        if (l == kHiddenLayers)
+        if (l > 0 && c->dwpart) {
+            DwTileArgs e;
+            e.n = c->n;
+            e.lg = c->lg;
+            e.npix = c->npix;
+            e.tiles = batch * (c->n / kDwRows) * (c->n / kDwCols);
+            const int per = (e.tiles + c->dw_wg - 1) / c->dw_wg;
+            e.per_wg = std::max(per, 8);
+            const int nwg = (e.tiles + e.per_wg - 1) / e.per_wg;
+            e.x = d.x;
+            e.dz = d.dz;
+            e.part = c->dwpart;
+            const int co = l == kHiddenLayers ? 1 : kHidden, nw = 9 * kHidden * co;
+            if (co == 1)
+                hipLaunchKernelGGL((k_conv_dw_tile<1>), dim3(nwg), dim3(384), 0, c->stream, e);
+            else
+                hipLaunchKernelGGL((k_conv_dw_tile<kHidden>), dim3(nwg), dim3(384), 0, c->stream, e);
+            hipLaunchKernelGGL(k_dw_reduce, dim3((nw + co + 63) / 64), dim3(256), 0, c->stream,
+                               nwg, nw + co, c->dwpart, d.dw, nw, d.db);
+        } else if (l == 0 && c->dwpart) {
+            DwTileArgs e;
+            e.n = c->n;
+            e.lg = c->lg;
+            e.npix = c->npix;
+            e.tiles = batch * (c->n / kDwRows) * (c->n / kDwCols);
+            const int per = (e.tiles + c->dw_wg - 1) / c->dw_wg;
+            e.per_wg = std::max(per, 8);
+            const int nwg = (e.tiles + e.per_wg - 1) / e.per_wg;
+            e.x = d.x;
+            e.dz = d.dz;
+            e.part = c->dwpart;
+            const int nw = 9 * kHidden;
+            hipLaunchKernelGGL((k_conv_dw_tile_in<kHidden>), dim3(nwg), dim3(256), 0, c->stream, e);
+            hipLaunchKernelGGL(k_dw_reduce, dim3((nw + kHidden + 63) / 64), dim3(256), 0,
+                               c->stream, 4 * nwg, nw + kHidden, c->dwpart, d.dw, nw, d.db);
+        } else if (l == kHiddenLayers)
             hipLaunchKernelGGL((k_conv_dw<kHidden, 1>), dim3(nblk), dim3(256), 0, c->stream, d);
         else if (l > 0)
             hipLaunchKernelGGL((k_conv_dw<kHidden, kHidden>), dim3(nblk), dim3(256), 0, c->stream, d);
@@ -598,8 +924,8 @@ int launch_backward(NsdClosure* c, int batch) {
         a.w = c->wb + c->bwoff[l];
         a.zprev = c->z[l - 1];
         a.dzprev = c->dz[l - 1];
-        if (l == kHiddenLayers) launch_conv<1, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
-        else launch_conv<kHidden, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
+        if (l == kHiddenLayers) launch_conv_lds<1, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
+        else launch_conv_lds<kHidden, kHidden, CONV_BWD_DATA>(a, batch, c->stream);
     }
     return ck(c, hipGetLastError(), "closure backward") ? 3 : 0;
 }
@@ -636,7 +962,7 @@ void nsd_closure_destroy(NsdClosure* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->prm, c->grad, c->adm, c->adv, c->widx, c->wb, c->x0, c->y, c->dy,
-                    c->xin, c->tin, c->loss, c->ds_x, c->ds_t, c->idx, c->sums};
+                    c->xin, c->tin, c->loss, c->ds_x, c->ds_t, c->idx, c->sums, c->dwpart};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int i = 0; i < kHiddenLayers; ++i)
@@ -686,6 +1012,11 @@ NsdClosure* nsd_closure_create(int n, int max_batch, unsigned long long seed, ch
     for (int i = 0; i < kHiddenLayers && ok; ++i)
         ok = al((void**)&c->z[i], 2 * P * kHidden) && al((void**)&c->act[i], 2 * P * kHidden) &&
              al((void**)&c->dz[i], 2 * P * kHidden);
+    if (ok && n % kDwCols == 0) {
+        const long tiles = (long)max_batch * (n / kDwRows) * (n / kDwCols);
+        c->dw_wg = (int)std::min<long>(kDwMaxWg, (tiles + 7) / 8);
+        ok = al((void**)&c->dwpart, sizeof(float) * (size_t)c->dw_wg * (9 * kHidden * kHidden + kHidden));
+    }
     if (!ok) return fail("closure: device allocation failed", c);
     // He-uniform initial weights from a SplitMix64 stream, zero biases
     std::vector<float> h(off, 0.f);

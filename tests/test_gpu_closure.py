@@ -50,7 +50,9 @@ def test_forward_matches_oracle(ns, n, batch):
     L.nsd_closure_destroy(h)
 
 
-@pytest.mark.parametrize("n,batch", [(16, 2), (32, 2), (64, 1)])
+# n = 16, 32: per-block weight-gradient kernel; n = 64, 128: the LDS-tiled one
+# (with 16 and 64 tiles per member) for the 32-channel-input layers
+@pytest.mark.parametrize("n,batch", [(16, 2), (32, 2), (64, 1), (128, 2)])
 def test_gradient_matches_oracle(ns, n, batch):
     L, h = make(ns, n, batch, seed=11)
     rng = np.random.default_rng(100 + n)
