@@ -276,7 +276,7 @@ def main():
                     help="skip the CPU baseline (one oracle step, about 20 s)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 step")
     ap.add_argument("--no-closure", action="store_true",
-                    help="skip training and evaluating the learned closure (about 150 s)")
+                    help="skip training and evaluating the learned closure (about 100 s)")
     ap.add_argument("--closure-snapshots", type=int, default=500,
                     help="training snapshots per fine run (64 runs) of the closure leg")
     ap.add_argument("--closure-train-only", action="store_true",

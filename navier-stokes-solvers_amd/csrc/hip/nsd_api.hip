@@ -816,7 +816,9 @@ namespace {
 size_t H_of(const NsdContext* c) { return (size_t)c->G.nkx * c->G.ny; }
 size_t R_of(const NsdContext* c) { return (size_t)c->G.nx * c->G.ny; }
 size_t C_of(const NsdContext* c) { return (size_t)c->G.nkx_keep * c->G.nky_keep; }
-size_t M_of(const NsdContext* c) { return (size_t)c->G.nkx_keep * c->G.ny; }
+// mixed-field and J buffers hold an even number of kx columns (J in 2 x 2
+// blocks pairs columns k and k ^ 1)
+size_t M_of(const NsdContext* c) { return (size_t)((c->G.nkx_keep + 1) & ~1) * c->G.ny; }
 
 int ensure_staging(NsdContext* c) {
     for (int i = 0; i < 4; ++i) {
@@ -1312,6 +1314,7 @@ int op_nonlinear(NsdContext* c, const double* w_ref, const double* psi_ref, doub
     launch_row_t<T>(c, r, c->G.ny / 2);
     ColArgs<T> f = col_args<T>(c);
     f.mode = COL_FWD;
+    f.jblocks = pairs_ok(c) ? 1 : 0;  // J of a pair-layout row stage
     f.out_ref = out_ref;
     f.fwd_scale = -1.0;  // spectral_core.c:108-111
     launch_col_t<T>(c, f, c->G.nkx);
@@ -1376,6 +1379,7 @@ int op_nl_member(NsdContext* c, int m, double* out_ref, int compute_rows) {
     ColArgs<T> f = col_args<T>(c);
     f.mode = COL_FWD;
     f.nl = (const cx<T>*)c->nl + (size_t)m * M_of(c);
+    f.jblocks = pairs_ok(c) ? 1 : 0;  // J of a pair-layout row stage
     f.out_ref = out_ref;
     f.fwd_scale = -1.0;  // spectral_core.c:108-111
     launch_col_t<T>(c, f, c->G.nkx);

@@ -317,6 +317,38 @@ __device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const 
     line_fft_c<DIR, LC>(v, lds, j, tw);
 }
 
+// Element i of a mixed field at (workgroup-uniform) base F, addressed as the
+// base plus a 32-bit byte offset: the loads then take the scalar base and one
+// 32-bit offset register per slot.  A mixed field is at most 2731 x 8192
+// FP64 complex values (358 MB), well inside 32 bits.
+// Element (k, y) of a mixed field (column k, row y): k ny + y through the
+// full-rate 24-bit multiply (k and ny are below 2^24; the 32-bit multiply
+// issues at a quarter of the rate).
+__device__ __forceinline__ unsigned col_elem(int k, int ny, int y) {
+    return __umul24((unsigned)k, (unsigned)ny) + (unsigned)y;
+}
+
+// Element (k, y) of J in 2 x 2 blocks: columns k, k ^ 1 and rows y, y ^ 1 in
+// 32 contiguous bytes at FP32, ordered (k even: y even, y odd; k odd: ...).
+// The row stage then stores one row pair's two values of a column as one
+// 16-byte piece next to its neighbour column's (a lane pair fills the block),
+// so no line leaves L2 half written; the column stage reads 8 of each 16
+// bytes.  The buffer holds an even number of columns.
+__device__ __forceinline__ unsigned jblock_elem(int k, int ny, int y) {
+    return ((__umul24((unsigned)(k >> 1), (unsigned)(ny >> 1)) + (unsigned)(y >> 1)) << 2) +
+           ((unsigned)(k & 1) << 1) + (unsigned)(y & 1);
+}
+
+template <typename T>
+__device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
+    return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
+                                          i * (unsigned)sizeof(cx<T>));
+}
+template <typename T>
+__device__ __forceinline__ cx<T>* at32(cx<T>* F, unsigned i) {
+    return reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(F) + i * (unsigned)sizeof(cx<T>));
+}
+
 // Kept bands of the 2/3 rule (spectral_initialization.c:46-48) for lines of
 // compile-time length LC: last kept kx of a row of length LC and last kept
 // |ky| of a column of length LC.  slot_kind classifies register slot s of
@@ -384,6 +416,7 @@ struct ColArgs {
     const cx<T>* nl;  // mixed input (COL_STAGE, COL_FWD)
     cx<T>* mixed[4];  // mixed outputs
     int pairs;        // COL_STAGE, COL_PRIME: write them in the pair layout (pair_layout)
+    int jblocks;      // COL_FWD: J in 2 x 2 blocks (written by a pair-layout row stage)
     cx<T>* state;     // compact state (read; written on the last stage)
     cx<T>* acc;       // compact RK accumulator
     cx<T>* stagebuf;  // compact stage state (classic scheme)
@@ -521,7 +554,9 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     constexpr int NB = line_regions<T, LC, NT, PR>();
     cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
-    const int kxi = blockIdx.x * lines + m;
+    // pair-layout kernels read J in 2 x 2 blocks: columns k and k ^ 1 share lines
+    const int kxi = (pair_layout<T, LC, NT, PR>() ? xcd_pair_remap(blockIdx.x, gridDim.x)
+                                                   : (int)blockIdx.x) * lines + m;
     const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
     const bool active = kxi < ncols;
     const bool has_data = active && kxi < G.nkx_keep;
@@ -547,8 +582,18 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     if constexpr (PF) {
         TwRegs<LC, T> tw;
         load_tw_regs<LC, 0>(tw, j, a.tw);
+        if constexpr (pair_layout<T, LC, NT, PR>()) {  // J in 2 x 2 blocks
+            // jblock_elem(kxi, ny, y), y = j + s TPL (TPL even): a column base,
+            // the lane's offset j + (j & ~1), and 2 TPL per slot
+            const unsigned jb0 = __umul24((unsigned)(kxi & ~1), (unsigned)G.ny) +
+                                 2u * (unsigned)(kxi & 1) + (unsigned)(j + (j & ~1));
 #pragma unroll
-        for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
+            for (int s = 0; s < P; ++s)
+                v[s] = has_data ? *at32(nl, jb0 + 2u * (unsigned)(s * TPL)) : zero;
+        } else {
+#pragma unroll
+            for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
+        }
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             const int kind = slot_kind(s, LC, Band<LC>::kym);
@@ -565,8 +610,14 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
         // the transform (it would wait there for them)
         __builtin_amdgcn_sched_barrier(0);
     } else if constexpr (MODE == COL_STAGE || MODE == COL_FWD) {
+        if (MODE == COL_FWD && a.jblocks) {
 #pragma unroll
-        for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
+            for (int s = 0; s < P; ++s)
+                v[s] = has_data ? *at32(nl, jblock_elem(kxi, G.ny, j + s * TPL)) : zero;
+        } else {
+#pragma unroll
+            for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
+        }
         fft_any<-1, LC>(v, lds, j, g, a.tw);
     }
 
@@ -1028,31 +1079,11 @@ __device__ __forceinline__ void build_two_rows2(cx<T> (&v1)[P], cx<T> (&v2)[P], 
     lds_read_c<LC>(v2, lds2, j);
 }
 
-// Element i of a mixed field at (workgroup-uniform) base F, addressed as the
-// base plus a 32-bit byte offset: the loads then take the scalar base and one
-// 32-bit offset register per slot.  A mixed field is at most 2731 x 8192
-// FP64 complex values (358 MB), well inside 32 bits.
-// Element (k, y) of a mixed field (column k, row y): k ny + y through the
-// full-rate 24-bit multiply (k and ny are below 2^24; the 32-bit multiply
-// issues at a quarter of the rate).
-__device__ __forceinline__ unsigned col_elem(int k, int ny, int y) {
-    return __umul24((unsigned)k, (unsigned)ny) + (unsigned)y;
-}
-
-template <typename T>
-__device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
-    return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
-                                          i * (unsigned)sizeof(cx<T>));
-}
-template <typename T>
-__device__ __forceinline__ cx<T>* at32(cx<T>* F, unsigned i) {
-    return reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(F) + i * (unsigned)sizeof(cx<T>));
-}
 
 // Forward x-transform of two real rows packed as v = ra + i rb, then
 // separation of the two half spectra and store of the kept columns into
 // the mixed layout at rows y and y+1.
-template <int LC, typename T, typename TW>
+template <int LC, typename T, typename TW, bool JB = false>
 __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j, const Line& g,
                                                const Geo& G, const TW& tw, cx<T>* out, int y,
                                                bool active) {
@@ -1073,9 +1104,17 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
             const cx<T> zm = lds[pad((Ll - k) & (Ll - 1))];
             const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
             const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
-            cx<T>* dst = at32(out, col_elem(k, G.ny, y));
-            dst[0] = ja;
-            dst[1] = jb;
+            if constexpr (JB) {
+                // jblock_elem(k, ny, y) for even y: (k & ~1) ny + 2 (k & 1) + 2 y
+                static_assert(sizeof(T) == 4, "J blocks are an FP32 layout");
+                const unsigned e = __umul24((unsigned)(k & ~1), (unsigned)G.ny) +
+                                   2u * (unsigned)((k & 1) + y);
+                *reinterpret_cast<float4*>(at32(out, e)) = float4{ja.x, ja.y, jb.x, jb.y};
+            } else {
+                cx<T>* dst = at32(out, col_elem(k, G.ny, y));
+                dst[0] = ja;
+                dst[1] = jb;
+            }
         }
     }
 }
@@ -1317,7 +1356,9 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
 #pragma unroll
             for (int s = 0; s < P; ++s)
                 v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
-            fwd_rows_store<LC>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
+            // a pair-layout row stage writes J in 2 x 2 blocks
+            fwd_rows_store<LC, T, TwRegs<LC, T>, pair_layout<T, LC, NT, PR>()>(
+                v, lds, j, g, G, tw, a.nl + moff, y0, active);
         }
         if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
