@@ -143,6 +143,72 @@ def test_subgrid_sample_matches_oracle(ns, oc):
     L.nsd_closure_destroy(net)
 
 
+def oracle_sample(oc, nf, nc, seed, k_peak):
+    """x = filtered vorticity and t = filter(N_fine) - N_coarse(x) in physical
+    space on the coarse grid, from the oracle's random-phase field, nonlinear
+    term and transforms."""
+    of = oc.Oracle(nf, nf, 2 * np.pi, 2 * np.pi, 1e3)
+    of.init_grids()
+    of.init_random_phase(seed, k_peak, 1.0)
+    wf = of.spec("omega_hat").copy()
+    of.velocity_from_vorticity()
+    of.nonlinear()
+    Nf = of.spec("nonlinear_hat").copy()
+    o = oc.Oracle(nc, nc, 2 * np.pi, 2 * np.pi, 1e3)
+    o.init_grids()
+    mask = o.mask.astype(bool)
+    kyc = np.where(np.arange(nc) <= nc // 2, np.arange(nc), np.arange(nc) - nc)
+
+    def filt(a):
+        rows = np.where(kyc >= 0, kyc, kyc + nf)
+        return np.where(mask, a[rows, : nc // 2 + 1], 0)
+
+    wbar = filt(wf)
+    o.spec("omega_hat")[:] = wbar
+    o.velocity_from_vorticity()
+    o.nonlinear()
+    return o.c2r(wbar), o.c2r(filt(Nf) - o.spec("nonlinear_hat"))
+
+
+def test_subgrid_sample_from_a_4096_fine_grid_vs_oracle(ns, oc):
+    """Samples from a 4096^2 fine grid against the oracle: FP64, and FP32,
+    where the row stage writes J in 2 x 2 blocks and the forward column
+    transform reads them so.  Outputs are FP32: the FP64 sample is held to
+    1e-6; the FP32 one keeps the vorticity to 1e-6 and the subgrid tendency, a
+    difference of two nonlinear terms each good to about 5e-7, to 1e-4 (a J
+    read in the wrong layout scrambles it at order 1)."""
+    # k_peak near the coarse cutoff (85), so that the subgrid tendency is of
+    # the order of the nonlinear term itself
+    nf, nc, seed, kp = 4096, 256, 8, 48.0
+    x_ref, t_ref = oracle_sample(oc, nf, nc, seed, kp)
+    L = ns.load()
+    err = C.create_string_buffer(256)
+    net = L.nsd_closure_create(nc, 1, 1, err, 256)
+    assert net and L.nsd_closure_dataset_reserve(net, 2) == 0
+    errs = []
+    for i, fp32 in enumerate((False, True)):
+        f = ctx_of(ns, nf, fp32)
+        f.random_phase(seed, k_peak=kp)
+        f.sync_device()
+        c = ctx_of(ns, nc, True)
+        c.sync_device()
+        px, pt = C.c_void_p(), C.c_void_p()
+        assert L.nsd_closure_dataset_slot(net, i, C.byref(px), C.byref(pt)) == 0
+        assert L.nsd_subgrid_sample(f.ctx, 0, c.ctx, px, pt) == 0, c.L.nsd_error(c.ctx)
+        x = np.zeros((nc, nc), np.float32)
+        t = np.zeros((nc, nc), np.float32)
+        assert L.nsd_closure_dataset_read(net, i, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                          t.ctypes.data_as(C.POINTER(C.c_float))) == 0
+        errs.append((rel_l2(x, x_ref), rel_l2(t, t_ref)))
+        print(f"4096^2 {'FP32' if fp32 else 'FP64'} fine grid vs oracle: vorticity "
+              f"{errs[-1][0]:.1e}, subgrid tendency {errs[-1][1]:.1e}", flush=True)
+        f.close()
+        c.close()
+    L.nsd_closure_destroy(net)
+    assert errs[0][0] <= 1e-6 and errs[0][1] <= 1e-6
+    assert errs[1][0] <= 1e-6 and errs[1][1] <= 1e-4
+
+
 def test_closure_source_enters_the_step(ns):
     """A network whose only non-zero parameter is the last bias b outputs the
     constant tendency b * out_scale; without drag the mean vorticity (the
