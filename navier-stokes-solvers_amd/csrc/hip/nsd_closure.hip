@@ -16,9 +16,11 @@
 // bf16) sits in LDS, transposed so each lane's 8 k-values are one 16-byte
 // read, and each lane's A values are one 16-byte load of 8 channels.  A workgroup stages B once and walks TILES tiles of 64
 // pixels, so the staging is paid once per 64 x TILES pixels.  Grids that are
-// multiples of 64 use the LDS-tiled kernels instead: k_conv_tile (forward and
-// input gradient) and k_conv_dw_tile / k_conv_dw_tile_in (weight gradients);
-// k_conv and k_conv_dw serve the smaller grids.
+// multiples of 64 use the LDS-tiled kernels instead: k_conv_tile2 (forward
+// layers with 32 input channels: persistent, the next patch streamed into LDS
+// while the current one is computed), k_conv_tile (the input layer and the
+// input gradients) and k_conv_dw_tile / k_conv_dw_tile_in (weight
+// gradients); k_conv and k_conv_dw serve the smaller grids.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -318,9 +320,176 @@ __global__ __launch_bounds__(512) void k_conv_tile(ConvArgs a) {
     }
 }
 
-// the tiled kernel where the grid is a multiple of its tile
+// 32-channel layers, persistent: two workgroups per CU walk the 8 x 32-pixel
+// tiles of all members.  A workgroup stages the layer's weights once (held
+// in registers, as in k_conv_tile) and keeps two LDS patches of 10 x 34
+// pixels: while it computes one tile from one patch, the next tile's patch
+// streams into the other with global_load_lds_dwordx4 (each lane's 16 bytes
+// land at the wave's LDS base + 16 x lane, which is the patch's own order of
+// 16-byte pieces), so no register holds it and one barrier per tile serves
+// both the reads of the old patch and the arrival of the new one.
+constexpr int kT2Cols = 32;
+
+__device__ __forceinline__ void g2lds16(const void* g, bf16* lds_wave_base) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave_base, 16,
+                                     0, 0);
+}
+
+template <int NN, int MODE>
+__global__ __launch_bounds__(512) void k_conv_tile2(ConvArgs a, int ntiles) {
+    constexpr int KC = 32;
+    constexpr int S = MODE == CONV_BWD_DATA ? -1 : 1;
+    constexpr int KP = 9 * KC;
+    constexpr int NP = (NN + 15) / 16 * 16;
+    constexpr int PR = kTileRows + 2, PC = kT2Cols + 2;
+    constexpr int NPC = PR * PC * (KC / 8);  // 16-byte pieces per patch
+    constexpr int PER = (NPC + 511) / 512;
+    __shared__ __align__(16) bf16 Bt[NP * KP];
+    // two distinct arrays (not one of two rows): the compiler's wait for the
+    // patch loads in flight then covers reads of the patch being loaded only
+    __shared__ __align__(16) bf16 X0[PR * PC * KC];
+    __shared__ __align__(16) bf16 X1[PR * PC * KC];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = a.n / kT2Cols, per_mem = tiles_x * (a.n / kTileRows);
+    const int nm = a.n - 1;
+    for (int e = tid; e < NP * KP; e += 512) {
+        const int nn = e / KP, k = e % KP;
+        bf16 v = 0;
+        if (nn < NN) {
+            const int tap = k / KC, kc = k % KC;
+            v = MODE == CONV_BWD_DATA ? a.w[(kc * 9 + tap) * NN + nn] : a.w[(nn * 9 + tap) * KC + kc];
+        }
+        Bt[e] = v;
+    }
+    // patch of tile t into buffer b: piece e = pixel (r PC + c), channel group e % 4
+    auto issue = [&](int t, bf16* Xd) {
+        const int mem = t / per_mem, rem = t - mem * per_mem;
+        const int r0 = (rem / tiles_x) * kTileRows, c0 = (rem % tiles_x) * kT2Cols;
+        const size_t boff = (size_t)mem * a.npix;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e0 = 512 * i + 64 * wave;  // this wave's first piece
+            if (e0 < NPC) {
+                const int e = e0 + lane;
+                if (e < NPC) {
+                    const int pix = e >> 2, q = e & 3;
+                    const int r = pix / PC, c = pix - r * PC;
+                    const int gy = (r0 + r - 1) & nm, gx = (c0 + c - 1) & nm;
+                    g2lds16(a.in + (boff + ((size_t)gy << a.lg) + gx) * KC + 8 * q, Xd + 8 * e0);
+                }
+            }
+        }
+    };
+    int t = blockIdx.x;
+    if (t < ntiles) issue(t, X0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int m = lane & 15, kq = lane >> 4;
+    s8 bv[KP / 32][NP / 16];
+#pragma unroll
+    for (int c = 0; c < KP / 32; ++c)
+#pragma unroll
+        for (int nb = 0; nb < NP / 16; ++nb)
+            bv[c][nb] = *reinterpret_cast<const s8*>(&Bt[(nb * 16 + m) * KP + c * 32 + 8 * kq]);
+    const int lr = wave;
+    // biases in registers: a global load in the loop would wait for the
+    // patch loads in flight (the vector memory counter retires in order)
+    float bias_r[NP / 16];
+#pragma unroll
+    for (int nb = 0; nb < NP / 16; ++nb)
+        bias_r[nb] = MODE == CONV_BWD_DATA || nb * 16 + m >= NN ? 0.f : a.bias[nb * 16 + m];
+    // one tile from patch X while the next tile's patch streams into Xn
+    auto step = [&](int t, const bf16* X, bf16* Xn) {
+        if (t + (int)gridDim.x < ntiles) issue(t + gridDim.x, Xn);
+        const int mem = t / per_mem, rem = t - mem * per_mem;
+        const int r0 = (rem / tiles_x) * kTileRows, c0 = (rem % tiles_x) * kT2Cols;
+        const size_t boff = (size_t)mem * a.npix;
+#pragma unroll 1
+        for (int blk = 0; blk < kT2Cols / 16; ++blk) {
+            const int lc = blk * 16;
+            f4 acc[NP / 16];
+#pragma unroll
+            for (int nb = 0; nb < NP / 16; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < KP / 32; ++c) {  // one tap of 32 channels per MFMA
+                const int tap = c, kc = 8 * kq;
+                const int pr = lr + 1 + S * (tap / 3 - 1), pc = lc + m + 1 + S * (tap % 3 - 1);
+                const s8 av = *reinterpret_cast<const s8*>(&X[(pr * PC + pc) * KC + kc]);
+#pragma unroll
+                for (int nb = 0; nb < NP / 16; ++nb) {
+                    if constexpr (MODE == CONV_FWD_LAST)  // D[channel][pixel]
+                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, bv[c][nb]), __builtin_bit_cast(bf8, av),
+                            acc[nb], 0, 0, 0);
+                    else  // D[pixel][channel]
+                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv[c][nb]),
+                            acc[nb], 0, 0, 0);
+                }
+            }
+            const size_t q0 = boff + ((size_t)(r0 + lr) << a.lg) + c0 + lc;
+            if constexpr (MODE == CONV_FWD_LAST) {
+                if (kq == 0) a.y[q0 + m] = (acc[0][0] + __shfl(bias_r[0], 0)) * a.yscale;
+            } else {
+#pragma unroll
+                for (int nb = 0; nb < NP / 16; ++nb) {
+                    const int nn = nb * 16 + m;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const size_t q = q0 + 4 * kq + i;
+                        if (nn < NN) {
+                            if constexpr (MODE == CONV_BWD_DATA) {
+                                a.dzprev[q * NN + nn] =
+                                    f2bf(acc[nb][i] * gelu_d(bf2f(a.zprev[q * NN + nn])));
+                            } else {
+                                const float zz = acc[nb][i] + bias_r[nb];
+                                if (a.z) a.z[q * NN + nn] = f2bf(zz);
+                                a.act[q * NN + nn] = f2bf(gelu(zz));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // the next patch has arrived and every read of this one is done
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    };
+    while (t < ntiles) {
+        step(t, X0, X1);
+        t += gridDim.x;
+        if (t >= ntiles) break;
+        step(t, X1, X0);
+        t += gridDim.x;
+    }
+}
+
+int conv_tile2_workgroups() {
+    static int wg = 0;
+    if (!wg) {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            wg = 2 * ncu;
+        if (wg <= 0) wg = 512;
+    }
+    return wg;
+}
+
+// the tiled kernels where the grid is a multiple of their tiles (the input
+// gradient stays on k_conv_tile: its epilogue loads the layer below's
+// pre-activations, which would wait behind the patch in flight)
 template <int KC, int NN, int MODE>
 void launch_conv_lds(const ConvArgs& a, int batch, hipStream_t st) {
+    if constexpr (KC == 32 && MODE != CONV_BWD_DATA) {
+        if (a.n % kT2Cols == 0 && a.n % kTileRows == 0) {
+            const int ntiles = (a.n / kT2Cols) * (a.n / kTileRows) * batch;
+            hipLaunchKernelGGL((k_conv_tile2<NN, MODE>),
+                               dim3(std::min(ntiles, conv_tile2_workgroups())), dim3(512), 0, st,
+                               a, ntiles);
+            return;
+        }
+    }
     if (a.n % kTileCols == 0)
         hipLaunchKernelGGL((k_conv_tile<KC, NN, MODE>),
                            dim3((a.n / kTileCols) * (a.n / kTileRows), batch), dim3(512), 0, st, a);
