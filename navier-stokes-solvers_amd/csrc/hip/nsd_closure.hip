@@ -394,10 +394,17 @@ __global__ __launch_bounds__(512) void k_conv_tile2(ConvArgs a, int ntiles) {
     const int lr = wave;
     // biases in registers: a global load in the loop would wait for the
     // patch loads in flight (the vector memory counter retires in order)
-    float bias_r[NP / 16];
+    // Both forward modes compute D[channel][pixel] (weights as the A
+    // operand): a lane then holds channels 4 kq .. 4 kq + 3 of one pixel, which
+    // leave as one 8-byte store.
+    f4 bias4[NP / 16];
 #pragma unroll
     for (int nb = 0; nb < NP / 16; ++nb)
-        bias_r[nb] = MODE == CONV_BWD_DATA || nb * 16 + m >= NN ? 0.f : a.bias[nb * 16 + m];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int nn = nb * 16 + 4 * kq + i;
+            bias4[nb][i] = MODE == CONV_BWD_DATA || nn >= NN ? 0.f : a.bias[nn];
+        }
     // one tile from patch X while the next tile's patch streams into Xn
     auto step = [&](int t, const bf16* X, bf16* Xn) {
         if (t + (int)gridDim.x < ntiles) issue(t + gridDim.x, Xn);
@@ -417,7 +424,7 @@ __global__ __launch_bounds__(512) void k_conv_tile2(ConvArgs a, int ntiles) {
                 const s8 av = *reinterpret_cast<const s8*>(&X[(pr * PC + pc) * KC + kc]);
 #pragma unroll
                 for (int nb = 0; nb < NP / 16; ++nb) {
-                    if constexpr (MODE == CONV_FWD_LAST)  // D[channel][pixel]
+                    if constexpr (MODE != CONV_BWD_DATA)  // D[channel][pixel]
                         acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                             __builtin_bit_cast(bf8, bv[c][nb]), __builtin_bit_cast(bf8, av),
                             acc[nb], 0, 0, 0);
@@ -429,7 +436,30 @@ __global__ __launch_bounds__(512) void k_conv_tile2(ConvArgs a, int ntiles) {
             }
             const size_t q0 = boff + ((size_t)(r0 + lr) << a.lg) + c0 + lc;
             if constexpr (MODE == CONV_FWD_LAST) {
-                if (kq == 0) a.y[q0 + m] = (acc[0][0] + __shfl(bias_r[0], 0)) * a.yscale;
+                // D[row = channel 4 kq + i][col = pixel m]: channel 0 in lanes 0..15
+                if (kq == 0) a.y[q0 + m] = (acc[0][0] + bias4[0][0]) * a.yscale;
+            } else if constexpr (MODE == CONV_FWD_GELU) {
+                const size_t q = q0 + m;  // this lane's pixel
+#pragma unroll
+                for (int nb = 0; nb < NP / 16; ++nb) {
+                    const int cb = nb * 16 + 4 * kq;  // its first channel
+                    if (cb < NN) {
+                        float zz[4], g[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            zz[i] = acc[nb][i] + bias4[nb][i];
+                            g[i] = gelu(zz[i]);
+                        }
+                        auto pack = [](float x0, float x1) {
+                            return (unsigned)f2bf(x0) | ((unsigned)f2bf(x1) << 16);
+                        };
+                        if (a.z)
+                            *reinterpret_cast<uint2*>(&a.z[q * NN + cb]) =
+                                uint2{pack(zz[0], zz[1]), pack(zz[2], zz[3])};
+                        *reinterpret_cast<uint2*>(&a.act[q * NN + cb]) =
+                            uint2{pack(g[0], g[1]), pack(g[2], g[3])};
+                    }
+                }
             } else {
 #pragma unroll
                 for (int nb = 0; nb < NP / 16; ++nb) {
@@ -441,10 +471,6 @@ __global__ __launch_bounds__(512) void k_conv_tile2(ConvArgs a, int ntiles) {
                             if constexpr (MODE == CONV_BWD_DATA) {
                                 a.dzprev[q * NN + nn] =
                                     f2bf(acc[nb][i] * gelu_d(bf2f(a.zprev[q * NN + nn])));
-                            } else {
-                                const float zz = acc[nb][i] + bias_r[nb];
-                                if (a.z) a.z[q * NN + nn] = f2bf(zz);
-                                a.act[q * NN + nn] = f2bf(gelu(zz));
                             }
                         }
                     }

@@ -31,8 +31,10 @@ def make(ns, n, batch, seed=3):
     return L, h
 
 
-# n = 16, 32: per-tile kernel; n = 64, 128: the LDS-tiled forward kernel
-@pytest.mark.parametrize("n,batch", [(16, 1), (32, 3), (64, 2), (128, 2)])
+# n = 16, 32: per-tile kernel; n = 64, 128: the LDS-tiled forward kernels; at
+# 128 x 16 the persistent one walks two tiles per workgroup on a 256-CU device,
+# the second patch loaded while the first is computed
+@pytest.mark.parametrize("n,batch", [(16, 1), (32, 3), (64, 2), (128, 2), (128, 16)])
 def test_forward_matches_oracle(ns, n, batch):
     L, h = make(ns, n, batch)
     rng = np.random.default_rng(n)
