@@ -442,12 +442,30 @@ __device__ __forceinline__ void cpass(cx<T> (&v)[P], int j, const TW& tw, cx<T>*
     }
 }
 
+// Butterfly of thread j in pass PASS: j itself, except in the stride-8 pass
+// of 4096-point FP32 lines.  There thread j = 8 a + k writes its eight
+// outputs at 68 a + k + 8 r (one-in-sixteen padding), so the two values of a
+// in a 16-lane write phase sit 8 banks apart and half their banks collide;
+// swapping bits 3 and 4 of j pairs a with a + 2 (136 elements, 16 banks
+// apart) and the phase covers 32 banks once.  The pass reads its inputs by
+// the same index, contiguous 32-lane groups either way; twiddles depend on
+// k = j mod 8 only.
+template <typename T, int L, int PASS>
+__device__ __forceinline__ int pass_lane(int j) {
+    if constexpr (sizeof(T) == 4 && L == 4096 && CLine<L>::stride(PASS) == 8 &&
+                  PASS + 1 < CLine<L>::passes)
+        return j ^ ((((j >> 3) ^ (j >> 4)) & 1) * 0x18);
+    else
+        return j;
+}
+
 template <int DIR, int L, int PASS, typename T, typename TW>
 __device__ __forceinline__ void cpasses(cx<T> (&v)[P], cx<T>* lds, int j, const TW& tw) {
     if constexpr (PASS < CLine<L>::passes) {
-        if constexpr (PASS > 0) lds_read_c<L>(v, lds, j);
+        const int jp = pass_lane<T, L, PASS>(j);
+        if constexpr (PASS > 0) lds_read_c<L>(v, lds, jp);
         __syncthreads();
-        cpass<DIR, L, PASS>(v, j, tw, lds);
+        cpass<DIR, L, PASS>(v, jp, tw, lds);
         if constexpr (PASS + 1 < CLine<L>::passes) __syncthreads();
         cpasses<DIR, L, PASS + 1>(v, lds, j, tw);
     }
@@ -545,11 +563,11 @@ __device__ __forceinline__ void cpasses2(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T>* 
                                          cx<T>* lds2, int j, const TW& tw) {
     if constexpr (PASS < CLine<L>::passes) {
         if constexpr (PASS > 0) {
-            lds_read_c<L>(v1, lds1, j);
-            lds_read_c<L>(v2, lds2, j);
+            lds_read_c<L>(v1, lds1, pass_lane<T, L, PASS>(j));
+            lds_read_c<L>(v2, lds2, pass_lane<T, L, PASS>(j));
         }
         __syncthreads();
-        cpass2<DIR, L, PASS>(v1, v2, j, tw, lds1, lds2);
+        cpass2<DIR, L, PASS>(v1, v2, pass_lane<T, L, PASS>(j), tw, lds1, lds2);
         if constexpr (PASS + 1 < CLine<L>::passes) __syncthreads();
         cpasses2<DIR, L, PASS + 1>(v1, v2, lds1, lds2, j, tw);
     }
