@@ -585,6 +585,204 @@ __device__ __forceinline__ void line_fft2_c(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T
     cpasses2<DIR, L, 0>(v1, v2, lds1, lds2, j, tw);
 }
 
+// ------------------------------------------------------------------------
+// 4096-point lines on 512 threads as 8 x 512 (wave-split transforms).
+//
+// Thread j = 64 w + l is lane l of wave w.  With n = n2 + 512 n1 and
+// k = k1 + 8 k2 the transform splits into 512 eight-point DFTs across the
+// waves and eight 512-point DFTs, one per wave:
+//   X[k1 + 8 k2] = sum_n2 W512^(n2 k2) W4096^(n2 k1) sum_n1 x[n2 + 512 n1] W8^(n1 k1).
+// Only the step between the two crosses waves.  The 512-point transform of
+// wave w runs on row w of the line region (512 elements, 544 with the
+// one-in-sixteen padding) in three radix-8 Stockham passes.  Its exchanges
+// stay inside the wave, whose LDS operations complete in order, so they need
+// a compiler fence and no workgroup barrier: two barriers per transform
+// remain instead of seven, and the cross-wave step reads and writes rows and
+// columns at unit stride.
+//
+// ws_a: standard pattern in (thread j holds x[j + 512 s]), wave-strided
+//       pattern out (lane l of wave w holds X[w + 8 l + 512 s]).
+// ws_b: wave-strided pattern in, standard pattern out: the 512-point DFTs of
+//       y[w + 8 m] first, then the cross-wave step with twiddle W4096^(j w).
+// A pointwise product between ws_a and ws_b transforms does not see the
+// pattern, which is how the row stage uses them.
+
+constexpr int WS_ROW = 544;  // padded row of 512 elements
+
+template <typename T>
+struct TwWs {
+    cx<T> wj;   // W4096^j: the cross-wave step
+    cx<T> w8;   // W64^(l mod 8): local pass of stride 8
+    cx<T> w64;  // W512^l: local pass of stride 64
+};
+
+// tw is the 4096-point table W4096^m
+template <typename T>
+__device__ __forceinline__ void load_tw_ws(TwWs<T>& t, int j, const cx<T>* __restrict__ tw) {
+    const int l = j & 63;
+    t.wj = tw[j];
+    t.w8 = tw[(l & 7) << 6];
+    t.w64 = tw[l << 3];
+}
+
+// W^1 .. W^7 from W by squaring and products (as in cpass)
+template <typename T>
+__device__ __forceinline__ void tw_powers8(cx<T> (&w)[8], cx<T> b) {
+    w[1] = b;
+    w[2] = cmul(b, b);
+    w[4] = cmul(w[2], w[2]);
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[1], w[4]);
+    w[6] = cmul(w[2], w[4]);
+    w[7] = cmul(w[3], w[4]);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void tw_apply8(cx<T> (&v)[P], const cx<T> (&w)[8]) {
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = DIR < 0 ? cmul(v[r], w[r]) : cmulc(v[r], w[r]);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8v(cx<T> (&v)[P]) {
+    dft8<DIR>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+}
+
+// A wave's LDS operations complete in the order issued: between its writes
+// and its reads of its own row only the compiler has to be kept from
+// reordering them.
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Butterfly of lane l in the stride-8 local pass: bits 3 and 4 swapped where
+// they differ.  Butterfly b = 8 a + k writes 64 a + k + 8 r, padded 68 a + k +
+// 8 r + r / 2, so a and a + 1 in one 16-lane write phase sit 8 banks apart;
+// a and a + 2 sit 16 apart and the phase covers the 32 banks once.
+__device__ __forceinline__ int ws_lane8(int l) { return l ^ ((((l >> 3) ^ (l >> 4)) & 1) * 0x18); }
+
+// NF rows (1 or 2) of the wave's 512-point transform, lane l holding elements
+// l + 64 s of each row in and out.  row[f] is this wave's row of field f; its
+// contents may be overwritten.
+template <int DIR, int NF, typename T>
+__device__ __forceinline__ void ws_local(cx<T> (*const (&v)[NF])[P], cx<T>* const (&row)[NF],
+                                         int l, const TwWs<T>& tw) {
+    // pass 0 (stride 1, no twiddles): butterfly l writes 8 l + r, which stays
+    // inside one 16-element padding group
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        dft8v<DIR>(*v[f]);
+        cx<T>* o = row[f] + padc(8 * l);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) o[r] = (*v[f])[r];
+    }
+    wave_lds_fence();
+    // pass 1 (stride 8)
+    {
+        const int b = ws_lane8(l);
+        const int k = b & 7;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const cx<T>* i = row[f] + padc(b);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) (*v[f])[s] = i[padc(64 * s)];
+        }
+        wave_lds_fence();
+        cx<T> w[8];
+        tw_powers8(w, tw.w8);
+        const int ob = padc((b - k) * 8) + k;  // padc(64 a + k) = 68 a + k
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            tw_apply8<DIR>(*v[f], w);
+            dft8v<DIR>(*v[f]);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) row[f][ob + 8 * r + (r >> 1)] = (*v[f])[r];
+        }
+    }
+    wave_lds_fence();
+    // pass 2 (stride 64, last): butterfly l; the result stays in registers
+    {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const cx<T>* i = row[f] + padc(l);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) (*v[f])[s] = i[padc(64 * s)];
+        }
+        wave_lds_fence();
+        cx<T> w[8];
+        tw_powers8(w, tw.w64);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            tw_apply8<DIR>(*v[f], w);
+            dft8v<DIR>(*v[f]);
+        }
+    }
+}
+
+// NF 4096-point transforms (regions lds[f]), standard pattern in,
+// wave-strided pattern out.  Every thread must be free to write the regions
+// on entry (the caller's barrier).  On return each wave still works in its
+// own rows: other waves may write the regions only after a workgroup barrier.
+template <int DIR, int NF, typename T>
+__device__ __forceinline__ void ws_a(cx<T> (*const (&v)[NF])[P], cx<T>* const (&lds)[NF], int j,
+                                     const TwWs<T>& tw) {
+    cx<T> w[8];
+    tw_powers8(w, tw.wj);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        dft8v<DIR>(*v[f]);
+        tw_apply8<DIR>(*v[f], w);
+        cx<T>* o = lds[f] + padc(j);  // u[k1] of column j to row k1
+#pragma unroll
+        for (int r = 0; r < 8; ++r) o[r * WS_ROW] = (*v[f])[r];
+    }
+    __syncthreads();
+    const int wv = j >> 6, l = j & 63;
+    cx<T>* row[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        row[f] = lds[f] + wv * WS_ROW;
+        const cx<T>* i = row[f] + padc(l);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) (*v[f])[s] = i[padc(64 * s)];
+    }
+    wave_lds_fence();
+    ws_local<DIR, NF>(v, row, l, tw);
+}
+
+// NF 4096-point transforms, wave-strided pattern in, standard pattern out.
+// On entry only the wave's own rows of the regions have to be free (as they
+// are right after ws_a on the same regions); on return other threads may
+// still be reading the regions.
+template <int DIR, int NF, typename T>
+__device__ __forceinline__ void ws_b(cx<T> (*const (&v)[NF])[P], cx<T>* const (&lds)[NF], int j,
+                                     const TwWs<T>& tw) {
+    const int wv = j >> 6, l = j & 63;
+    cx<T>* row[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) row[f] = lds[f] + wv * WS_ROW;
+    ws_local<DIR, NF>(v, row, l, tw);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        cx<T>* o = row[f] + padc(l);  // Y_w[l + 64 s] to row w
+#pragma unroll
+        for (int s = 0; s < 8; ++s) o[padc(64 * s)] = (*v[f])[s];
+    }
+    __syncthreads();
+    cx<T> w[8];
+    tw_powers8(w, tw.wj);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const cx<T>* i = lds[f] + padc(j);  // column j of every row
+#pragma unroll
+        for (int r = 0; r < 8; ++r) (*v[f])[r] = i[r * WS_ROW];
+        tw_apply8<DIR>(*v[f], w);
+        dft8v<DIR>(*v[f]);
+    }
+}
+
 // XCD-aware block order: MI355X dispatches workgroup b to XCD b % 8; this
 // gives each XCD a contiguous range of logical tiles so that neighbouring
 // tiles (which share cache lines of the transposed intermediates) meet in

@@ -316,6 +316,18 @@ __device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const 
                                         const TwRegs<LC, T>& tw) {
     line_fft_c<DIR, LC>(v, lds, j, tw);
 }
+// wave-split 4096-point transform, wave-strided pattern in (a pointwise
+// product of ws_a outputs), standard pattern out; the region is free again
+// only after a workgroup barrier
+template <int DIR, int LC, typename T>
+__device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const Line&,
+                                        const TwWs<T>& tw) {
+    static_assert(LC == 4096, "wave-split transforms are 4096-point");
+    cx<T>(*const vv[1])[P] = {&v};
+    cx<T>* const ll[1] = {lds};
+    ws_b<DIR, 1>(vv, ll, j, tw);
+    __syncthreads();
+}
 
 // Element i of a mixed field at (workgroup-uniform) base F, addressed as the
 // base plus a 32-bit byte offset: the loads then take the scalar base and one
@@ -1306,6 +1318,59 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     // accumulators live across transforms.
     cx<T> X[P], Jp[P];
     SpeedMax<T> um;
+    if constexpr (pair_layout<T, LC, NT, PR>()) {
+        // FP32 rows of 4096 on 512 threads: the paired gather-ahead schedule
+        // below with wave-split transforms (ws_a for the four fields, whose
+        // outputs are in the wave-strided pattern, ws_b for J, which takes
+        // that pattern back to the standard one): 11 workgroup barriers per
+        // row pair instead of 27.
+        TwWs<T> tw;
+        load_tw_ws(tw, j, a.tw);
+        if (blk * lines < G.ny / 2) {
+            RowPairRegs<T> r0, r1, q0, q1;
+            const bool pl = a.pairs != 0;
+            if (pl) gather_pair_rows<LC>(r0, r1, j, G, a.mixed[0] + 2 * moff, y0, active);
+            else {
+                gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
+                gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
+            }
+            stage_two_rows_regs<LC>(lds, j, r0);
+            stage_two_rows_regs<LC>(lds2, j, r1);
+            if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
+            else {
+                gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
+                gather_two_rows<LC>(q1, j, G, a.mixed[3] + moff, y0, active);
+            }
+            __syncthreads();
+            cx<T> v2[P];
+            cx<T>(*const vv[2])[P] = {&v, &v2};
+            cx<T>* const ll[2] = {lds, lds2};
+            lds_read_c<LC>(v, lds, j);
+            lds_read_c<LC>(v2, lds2, j);
+            __syncthreads();
+            ws_a<+1, 2>(vv, ll, j, tw);  // psi_x, psi_y
+            if (a.track_umax) add_speeds(um, v, v2);
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                X[s] = v[s];
+                Jp[s] = v2[s];
+            }
+            __syncthreads();
+            stage_two_rows_regs<LC>(lds, j, q0);
+            stage_two_rows_regs<LC>(lds2, j, q1);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            lds_read_c<LC>(v2, lds2, j);
+            __syncthreads();
+            ws_a<+1, 2>(vv, ll, j, tw);  // omega_x, omega_y
+#pragma unroll
+            for (int s = 0; s < P; ++s)
+                v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
+            fwd_rows_store<LC, T, TwWs<T>, true>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
+        }
+        if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        return;
+    }
     if constexpr (paired_lines<T, LC, NT, PR>() && row_prefetch<T, LC>()) {
         // Twiddles first, then the psi gathers, then the omega gathers,
         // which stay in flight while the psi fields are transformed: with
