@@ -388,6 +388,19 @@ constexpr int line_regions() {
     return paired_lines<T, LC, NT, PR>() ? 2 : 1;
 }
 
+// FP64 rows of 4096 on 512 threads (one workgroup per CU): wave-split
+// transforms, with the staged lines in one region and the cross-wave rows in
+// a second one, so that staging the next field needs no barrier of its own.
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
+constexpr bool ws_rows64() {
+    return sizeof(T) == 8 && LC == 4096 && NT == 512 && !paired_lines<T, LC, NT, PR>();
+}
+// line regions of the row-stage kernel
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
+constexpr int row_regions() {
+    return ws_rows64<T, LC, NT, PR>() ? 2 : line_regions<T, LC, NT, PR>();
+}
+
 constexpr int slot_kind(int s, int L, int km) {
     const int tpl = L / P, lo = s * tpl, hi = lo + tpl - 1;
     if (hi <= km) return 1;
@@ -1245,7 +1258,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    constexpr int NB = line_regions<T, LC, NT, PR>();
+    constexpr int NB = row_regions<T, LC, NT, PR>();
     cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -1445,6 +1458,57 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
         for (int s = 0; s < P; ++s)
             v[s] = mk(X[s].x * v2[s].x - Jp[s].x * v[s].x, X[s].y * v2[s].y - Jp[s].y * v[s].y);
         fwd_rows_store<LC>(v, lds, j, g, G, a.tw, a.nl + moff, y0, active);
+        if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
+        return;
+    }
+    if constexpr (ws_rows64<T, LC, NT, PR>()) {
+        // FP64 rows of 4096: one field at a time, each gather in flight while
+        // the previous field is transformed (as below), with wave-split
+        // transforms whose cross-wave rows live in lds2.  Every read of the
+        // staged line in lds precedes the barrier inside ws_a, so the next
+        // field is staged without one: 10 barriers per row pair instead of 45.
+        TwWs<T> tw;
+        load_tw_ws(tw, j, a.tw);
+        if (blk * lines < G.ny / 2) {
+            cx<T>(*const vv[1])[P] = {&v};
+            cx<T>* const rr[1] = {lds2};
+            RowPairRegs<T> r;
+            gather_two_rows<LC>(r, j, G, a.mixed[0] + moff, y0, active);
+            stage_two_rows_regs<LC>(lds, j, r);
+            gather_two_rows<LC>(r, j, G, a.mixed[1] + moff, y0, active);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            ws_a<+1, 1>(vv, rr, j, tw);  // psi_x
+#pragma unroll
+            for (int s = 0; s < P; ++s) X[s] = v[s];
+            __builtin_amdgcn_sched_barrier(0);
+            stage_two_rows_regs<LC>(lds, j, r);
+            gather_two_rows<LC>(r, j, G, a.mixed[2] + moff, y0, active);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            ws_a<+1, 1>(vv, rr, j, tw);  // psi_y
+            if (a.track_umax) add_speeds(um, X, v);
+#pragma unroll
+            for (int s = 0; s < P; ++s) Jp[s] = v[s];
+            __builtin_amdgcn_sched_barrier(0);
+            stage_two_rows_regs<LC>(lds, j, r);
+            gather_two_rows<LC>(r, j, G, a.mixed[3] + moff, y0, active);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            ws_a<+1, 1>(vv, rr, j, tw);  // omega_x
+#pragma unroll
+            for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
+            __builtin_amdgcn_sched_barrier(0);
+            stage_two_rows_regs<LC>(lds, j, r);
+            __syncthreads();
+            lds_read_c<LC>(v, lds, j);
+            ws_a<+1, 1>(vv, rr, j, tw);  // omega_y
+#pragma unroll
+            for (int s = 0; s < P; ++s)
+                v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
+            // J: ws_b in lds, whose rows nobody reads any more
+            fwd_rows_store<LC, T, TwWs<T>>(v, lds, j, g, G, tw, a.nl + moff, y0, active);
+        }
         if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }

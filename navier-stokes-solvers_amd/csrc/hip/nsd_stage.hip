@@ -43,6 +43,15 @@ size_t var_lds() {
     return (size_t)lines * line_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
 }
 
+// the row stage's LDS (FP64 rows of 4096 take a second line region)
+template <typename T, int L, int V>
+size_t row_lds() {
+    constexpr int NT = var_nt<L, V>();
+    constexpr int lines = NT * P / L;
+    constexpr int PR = var_pr<V>();
+    return (size_t)lines * row_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
+}
+
 // ERR: the error controller's instantiations (launch variant 0 only)
 template <typename T, int LOG, int V, bool ERR = false>
 void row_stage_LV(const RowArgs<T>& a, int items, int members, hipStream_t st) {
@@ -52,7 +61,7 @@ void row_stage_LV(const RowArgs<T>& a, int items, int members, hipStream_t st) {
         constexpr int PR = var_pr<V>();
         constexpr int lines = NT * P / L;
         const int blocks = (items + lines - 1) / lines;
-        const size_t lds = var_lds<T, L, V>();
+        const size_t lds = row_lds<T, L, V>();
         hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L, PR, ERR>), dim3(blocks, members), dim3(NT),
                            lds, st, a);
     }
@@ -113,7 +122,8 @@ int var_pairs() {
 template <typename T, int LOG, int V>
 int var_valid(int maxlds) {
     constexpr int L = 1 << LOG;
-    if constexpr (var_ok<L, V>()) return var_lds<T, L, V>() <= (size_t)maxlds ? 1 : 0;
+    if constexpr (var_ok<L, V>())
+        return std::max(var_lds<T, L, V>(), row_lds<T, L, V>()) <= (size_t)maxlds ? 1 : 0;
     return 0;
 }
 
