@@ -783,6 +783,45 @@ __device__ __forceinline__ void ws_b(cx<T> (*const (&v)[NF])[P], cx<T>* const (&
     }
 }
 
+// Wave-strided pattern to standard pattern through region tr.  Wave w writes
+// element c = l + 64 s of its transform at 544 w + (c + 4 w) mod 512 (its own
+// row: the skew by 4 w puts the eight rows of one column in different banks,
+// and each 16-lane write phase stays on 16 consecutive elements mod 512);
+// after a barrier thread j reads X[j + 512 s] from row j mod 8, column
+// j / 8 + 64 s, 32 lanes on 32 distinct banks pairs.  On entry the wave's own
+// row of tr must be free; on return other threads may still read tr.
+template <int NF, typename T>
+__device__ __forceinline__ void ws_transpose(cx<T> (*const (&v)[NF])[P], cx<T>* const (&tr)[NF],
+                                             int j) {
+    const int wv = j >> 6, l = j & 63;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        cx<T>* o = tr[f] + wv * WS_ROW;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) o[(l + 4 * wv + 64 * s) & 511] = (*v[f])[s];
+    }
+    __syncthreads();
+    const int rw = j & 7, c0 = j >> 3;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const cx<T>* i = tr[f] + rw * WS_ROW;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) (*v[f])[s] = i[(c0 + 4 * rw + 64 * s) & 511];
+    }
+}
+
+// NF 4096-point transforms, standard pattern in and out: ws_a on regions
+// rows, then the transposing exchange through regions tr (which may be the
+// same regions: each wave writes only its own row there).  Entry condition
+// as for ws_a on rows and, when tr differs, every thread free to write tr;
+// on return other threads may still read tr and work in their rows.
+template <int DIR, int NF, typename T>
+__device__ __forceinline__ void ws_c(cx<T> (*const (&v)[NF])[P], cx<T>* const (&rows)[NF],
+                                     cx<T>* const (&tr)[NF], int j, const TwWs<T>& tw) {
+    ws_a<DIR, NF>(v, rows, j, tw);
+    ws_transpose<NF>(v, tr, j);
+}
+
 // XCD-aware block order: MI355X dispatches workgroup b to XCD b % 8; this
 // gives each XCD a contiguous range of logical tiles so that neighbouring
 // tiles (which share cache lines of the transposed intermediates) meet in

@@ -26,6 +26,8 @@
 //                 state and their inverse y-transforms.
 #pragma once
 
+#include <type_traits>
+
 #include "nsd_fft.hpp"
 
 namespace nsd {
@@ -388,17 +390,20 @@ constexpr int line_regions() {
     return paired_lines<T, LC, NT, PR>() ? 2 : 1;
 }
 
-// FP64 rows of 4096 on 512 threads (one workgroup per CU): wave-split
-// transforms, with the staged lines in one region and the cross-wave rows in
-// a second one, so that staging the next field needs no barrier of its own.
+// FP64 lines of 4096 on 512 threads (one workgroup per CU): wave-split
+// transforms with a second line region.  The row stage stages each field in
+// one region and runs the cross-wave rows in the other, so staging the next
+// field needs no barrier of its own; the column stage transposes back to the
+// standard pattern through the second region, so consecutive transforms need
+// none between them.
 template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
-constexpr bool ws_rows64() {
+constexpr bool ws_lines64() {
     return sizeof(T) == 8 && LC == 4096 && NT == 512 && !paired_lines<T, LC, NT, PR>();
 }
-// line regions of the row-stage kernel
+// line regions of the stage kernels
 template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
-constexpr int row_regions() {
-    return ws_rows64<T, LC, NT, PR>() ? 2 : line_regions<T, LC, NT, PR>();
+constexpr int stage_regions() {
+    return ws_lines64<T, LC, NT, PR>() ? 2 : line_regions<T, LC, NT, PR>();
 }
 
 constexpr int slot_kind(int s, int L, int km) {
@@ -576,9 +581,10 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    constexpr int NB = line_regions<T, LC, NT, PR>();
+    constexpr int NB = stage_regions<T, LC, NT, PR>();
     cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
+    constexpr bool WS = ws_lines64<T, LC, NT, PR>();
     // pair-layout kernels read J in 2 x 2 blocks: columns k and k ^ 1 share lines
     const int kxi = (pair_layout<T, LC, NT, PR>() ? xcd_pair_remap(blockIdx.x, gridDim.x)
                                                    : (int)blockIdx.x) * lines + m;
@@ -605,8 +611,9 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
 
     // ---- forward y-transform of the mixed nonlinear field
     if constexpr (PF) {
-        TwRegs<LC, T> tw;
-        load_tw_regs<LC, 0>(tw, j, a.tw);
+        std::conditional_t<WS, TwWs<T>, TwRegs<LC, T>> tw;
+        if constexpr (WS) load_tw_ws(tw, j, a.tw);
+        else load_tw_regs<LC, 0>(tw, j, a.tw);
         if constexpr (pair_layout<T, LC, NT, PR>()) {  // J in 2 x 2 blocks
             // jblock_elem(kxi, ny, y), y = j + s TPL (TPL even): a column base,
             // the lane's offset j + (j & ~1), and 2 TPL per slot
@@ -630,7 +637,14 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             pw0[s] = state[ci];
             pacc[s] = a.stage > 0 ? acc[ci] : zero;
         }
-        fft_any<-1, LC>(v, lds, j, g, tw);
+        if constexpr (WS) {
+            cx<T>(*const vv[1])[P] = {&v};
+            cx<T>* const r1[1] = {lds};
+            cx<T>* const t1[1] = {lds2};
+            ws_c<-1, 1>(vv, r1, t1, j, tw);
+        } else {
+            fft_any<-1, LC>(v, lds, j, g, tw);
+        }
         // nothing that uses the values loaded above may be scheduled into
         // the transform (it would wait there for them)
         __builtin_amdgcn_sched_barrier(0);
@@ -941,6 +955,37 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                     o1[mbase + j + s * TPL] = v[s];
                     o2[mbase + j + s * TPL] = v2[s];
                 }
+            }
+        }
+    } else if constexpr (WS) {
+        // FP64 columns of 4096: the four outputs one at a time through ws_c
+        // (spectral_core.c:62-99), rows in lds, the transposing exchange in
+        // lds2: two barriers per transform
+        TwWs<T> tw;
+        load_tw_ws(tw, j, a.tw);
+        cx<T>(*const vv[1])[P] = {&v};
+        cx<T>* const r1[1] = {lds};
+        cx<T>* const t1[1] = {lds2};
+#pragma unroll 1
+        for (int f = 3; f >= 0; --f) {
+            if (f == 1) {
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
+                    const T k2 = kx * kx + ky * ky;
+                    w[s] = k2 > T(1e-12) ? neg_inv_k2(k2) * w[s] : zero;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
+                v[s] = mul_i(w[s], (f & 1) ? ky : kx);
+            }
+            ws_c<+1, 1>(vv, r1, t1, j, tw);
+            if (has_data) {
+                cx<T>* out = a.mixed[f] + mixed_off;
+#pragma unroll
+                for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
     } else {
@@ -1258,7 +1303,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     const int Ll = LC > 0 ? LC : g.L;
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
-    constexpr int NB = row_regions<T, LC, NT, PR>();
+    constexpr int NB = stage_regions<T, LC, NT, PR>();
     cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
     cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -1461,7 +1506,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
         if (a.track_umax) block_max_to(active ? double(um.value()) : 0.0, &a.sc[blockIdx.y].umax2_bits, red);
         return;
     }
-    if constexpr (ws_rows64<T, LC, NT, PR>()) {
+    if constexpr (ws_lines64<T, LC, NT, PR>()) {
         // FP64 rows of 4096: one field at a time, each gather in flight while
         // the previous field is transformed (as below), with wave-split
         // transforms whose cross-wave rows live in lds2.  Every read of the

@@ -35,21 +35,13 @@ constexpr bool var_ok() {
     return V == 0 || (L >= 64 && L <= 1024 && NT <= 1024 && NT >= L / P && NT >= 64);
 }
 
+// the stage kernels' LDS (FP64 lines of 4096 take a second line region)
 template <typename T, int L, int V>
-size_t var_lds() {
+size_t stage_lds() {
     constexpr int NT = var_nt<L, V>();
     constexpr int lines = NT * P / L;
     constexpr int PR = var_pr<V>();
-    return (size_t)lines * line_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
-}
-
-// the row stage's LDS (FP64 rows of 4096 take a second line region)
-template <typename T, int L, int V>
-size_t row_lds() {
-    constexpr int NT = var_nt<L, V>();
-    constexpr int lines = NT * P / L;
-    constexpr int PR = var_pr<V>();
-    return (size_t)lines * row_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
+    return (size_t)lines * stage_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
 }
 
 // ERR: the error controller's instantiations (launch variant 0 only)
@@ -61,7 +53,7 @@ void row_stage_LV(const RowArgs<T>& a, int items, int members, hipStream_t st) {
         constexpr int PR = var_pr<V>();
         constexpr int lines = NT * P / L;
         const int blocks = (items + lines - 1) / lines;
-        const size_t lds = row_lds<T, L, V>();
+        const size_t lds = stage_lds<T, L, V>();
         hipLaunchKernelGGL((k_row<T, NT, ROW_STAGE, L, PR, ERR>), dim3(blocks, members), dim3(NT),
                            lds, st, a);
     }
@@ -76,7 +68,7 @@ void col_fused_LV(const ColArgs<T>& a, int ncols, int members, hipStream_t st) {
         constexpr int lines = NT * P / L;
         const int blocks = (ncols + lines - 1) / lines;
         const dim3 grid(blocks, members);
-        const size_t lds = var_lds<T, L, V>();
+        const size_t lds = stage_lds<T, L, V>();
         if (a.mode == COL_PRIME)
             hipLaunchKernelGGL((k_col<T, NT, COL_PRIME, 0, L, PR>), grid, dim3(NT), lds, st, a);
         else if (a.phys.scheme == 0)
@@ -123,7 +115,7 @@ template <typename T, int LOG, int V>
 int var_valid(int maxlds) {
     constexpr int L = 1 << LOG;
     if constexpr (var_ok<L, V>())
-        return std::max(var_lds<T, L, V>(), row_lds<T, L, V>()) <= (size_t)maxlds ? 1 : 0;
+        return stage_lds<T, L, V>() <= (size_t)maxlds ? 1 : 0;
     return 0;
 }
 
