@@ -1161,8 +1161,20 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
     const int Ll = LC > 0 ? LC : g.L;
     const int kxm = LC > 0 ? Band<LC>::kxm : G.kxmax;
     fft_any<-1, LC>(v, lds, j, g, tw);
-    if constexpr (LC > 0) lds_write_c<LC>(v, lds, j);
-    else lds_write_std(v, lds, j, TPL);
+    if constexpr (LC > 0) {
+        // the separation below reads kept columns k and their partners
+        // Ll - k only, never a slot inside the dealiased band (kind 0)
+        if constexpr (LC / P % 16 == 0) {
+            cx<T>* b = lds + pad(j);
+#pragma unroll
+            for (int s = 0; s < P; ++s)
+                if (slot_kind(s, LC, Band<LC>::kxm) != 0) b[padc(s * TPL)] = v[s];
+        } else {
+            lds_write_c<LC>(v, lds, j);
+        }
+    } else {
+        lds_write_std(v, lds, j, TPL);
+    }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < P; ++s) {
@@ -1187,6 +1199,19 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
             }
         }
     }
+}
+
+// Standard-pattern read of a line staged with SKIP0: its kind-0 slots are
+// zero in registers, known to the compiler, so the first butterflies fold
+// them away, and they cost no LDS read.
+template <int LC, typename T>
+__device__ __forceinline__ void lds_read_kept(cx<T> (&v)[P], const cx<T>* lds, int j) {
+    constexpr int TPL = LC / P;
+    static_assert(TPL % 16 == 0, "compile-time offsets need TPL % 16 == 0");
+    const cx<T>* b = lds + pad(j);
+#pragma unroll
+    for (int s = 0; s < P; ++s)
+        v[s] = slot_kind(s, LC, Band<LC>::kxm) == 0 ? mk(T(0), T(0)) : b[padc(s * TPL)];
 }
 
 // Kept columns of one field at rows y, y + 1 (the two rows of one packed
@@ -1244,8 +1269,10 @@ __device__ __forceinline__ void gather_pair_rows(RowPairRegs<float>& f0, RowPair
     }
 }
 
-// The LDS image of stage_two_rows, from gathered registers.
-template <int LC, typename T>
+// The LDS image of stage_two_rows, from gathered registers.  SKIP0: the
+// slots that lie inside the dealiased band for every thread (kind 0) are
+// left unwritten, for a reader that takes them as zero (lds_read_kept).
+template <int LC, bool SKIP0 = false, typename T>
 __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const RowPairRegs<T>& r) {
     constexpr int TPL = LC / P, kxm = Band<LC>::kxm;
     const cx<T> zero = mk(T(0), T(0));
@@ -1255,7 +1282,7 @@ __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const Row
         const int k = j + s * TPL;
         if (kind == 2) continue;  // written by the owner of LC - k
         if (kind == 0) {
-            lds[pad(k)] = zero;
+            if (!SKIP0) lds[pad(k)] = zero;
             continue;
         }
         if (kind == 1 || k <= kxm) {
@@ -1392,8 +1419,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
                 gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
                 gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
             }
-            stage_two_rows_regs<LC>(lds, j, r0);
-            stage_two_rows_regs<LC>(lds2, j, r1);
+            stage_two_rows_regs<LC, true>(lds, j, r0);
+            stage_two_rows_regs<LC, true>(lds2, j, r1);
             if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
             else {
                 gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
@@ -1403,8 +1430,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             cx<T> v2[P];
             cx<T>(*const vv[2])[P] = {&v, &v2};
             cx<T>* const ll[2] = {lds, lds2};
-            lds_read_c<LC>(v, lds, j);
-            lds_read_c<LC>(v2, lds2, j);
+            lds_read_kept<LC>(v, lds, j);
+            lds_read_kept<LC>(v2, lds2, j);
             __syncthreads();
             ws_a<+1, 2>(vv, ll, j, tw);  // psi_x, psi_y
             if (a.track_umax) add_speeds(um, v, v2);
@@ -1414,11 +1441,11 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
                 Jp[s] = v2[s];
             }
             __syncthreads();
-            stage_two_rows_regs<LC>(lds, j, q0);
-            stage_two_rows_regs<LC>(lds2, j, q1);
+            stage_two_rows_regs<LC, true>(lds, j, q0);
+            stage_two_rows_regs<LC, true>(lds2, j, q1);
             __syncthreads();
-            lds_read_c<LC>(v, lds, j);
-            lds_read_c<LC>(v2, lds2, j);
+            lds_read_kept<LC>(v, lds, j);
+            lds_read_kept<LC>(v2, lds2, j);
             __syncthreads();
             ws_a<+1, 2>(vv, ll, j, tw);  // omega_x, omega_y
 #pragma unroll
@@ -1519,34 +1546,34 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             cx<T>* const rr[1] = {lds2};
             RowPairRegs<T> r;
             gather_two_rows<LC>(r, j, G, a.mixed[0] + moff, y0, active);
-            stage_two_rows_regs<LC>(lds, j, r);
+            stage_two_rows_regs<LC, true>(lds, j, r);
             gather_two_rows<LC>(r, j, G, a.mixed[1] + moff, y0, active);
             __syncthreads();
-            lds_read_c<LC>(v, lds, j);
+            lds_read_kept<LC>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // psi_x
 #pragma unroll
             for (int s = 0; s < P; ++s) X[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
-            stage_two_rows_regs<LC>(lds, j, r);
+            stage_two_rows_regs<LC, true>(lds, j, r);
             gather_two_rows<LC>(r, j, G, a.mixed[2] + moff, y0, active);
             __syncthreads();
-            lds_read_c<LC>(v, lds, j);
+            lds_read_kept<LC>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // psi_y
             if (a.track_umax) add_speeds(um, X, v);
 #pragma unroll
             for (int s = 0; s < P; ++s) Jp[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
-            stage_two_rows_regs<LC>(lds, j, r);
+            stage_two_rows_regs<LC, true>(lds, j, r);
             gather_two_rows<LC>(r, j, G, a.mixed[3] + moff, y0, active);
             __syncthreads();
-            lds_read_c<LC>(v, lds, j);
+            lds_read_kept<LC>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // omega_x
 #pragma unroll
             for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
             __builtin_amdgcn_sched_barrier(0);
-            stage_two_rows_regs<LC>(lds, j, r);
+            stage_two_rows_regs<LC, true>(lds, j, r);
             __syncthreads();
-            lds_read_c<LC>(v, lds, j);
+            lds_read_kept<LC>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // omega_y
 #pragma unroll
             for (int s = 0; s < P; ++s)
