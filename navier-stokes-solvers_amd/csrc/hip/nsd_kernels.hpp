@@ -1214,6 +1214,12 @@ __device__ __forceinline__ void lds_read_kept(cx<T> (&v)[P], const cx<T>* lds, i
         v[s] = slot_kind(s, LC, Band<LC>::kxm) == 0 ? mk(T(0), T(0)) : b[padc(s * TPL)];
 }
 
+template <int LC, bool KEPT, typename T>
+__device__ __forceinline__ void lds_read_any(cx<T> (&v)[P], const cx<T>* lds, int j) {
+    if constexpr (KEPT) lds_read_kept<LC>(v, lds, j);
+    else lds_read_c<LC>(v, lds, j);
+}
+
 // Kept columns of one field at rows y, y + 1 (the two rows of one packed
 // line) for this thread's slots, held in registers from the gather to the
 // LDS staging.  Zero for an inactive line.
@@ -1463,6 +1469,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
         // that would have to wait behind the gathers.
         TwRegs<LC, T> tw;
         load_tw_regs<LC, 0>(tw, j, a.tw);
+        // slots inside the dealiased band neither staged nor read back
+        constexpr bool KEPT = LC / P % 16 == 0;
         // (always true: the grid holds no idle workgroup; as a branch it keeps
         // the compiler's schedule of the loads above within 128 VGPRs)
         if (blk * lines < G.ny / 2) {
@@ -1476,8 +1484,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
                 gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
                 gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
             }
-            stage_two_rows_regs<LC>(lds, j, r0);
-            stage_two_rows_regs<LC>(lds2, j, r1);
+            stage_two_rows_regs<LC, KEPT>(lds, j, r0);
+            stage_two_rows_regs<LC, KEPT>(lds2, j, r1);
             if constexpr (pair_layout<T, LC, NT, PR>()) {
                 if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
             }
@@ -1487,8 +1495,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             }
             __syncthreads();
             cx<T> v2[P];
-            lds_read_c<LC>(v, lds, j);
-            lds_read_c<LC>(v2, lds2, j);
+            lds_read_any<LC, KEPT>(v, lds, j);
+            lds_read_any<LC, KEPT>(v2, lds2, j);
             line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_x, psi_y
             if (a.track_umax) add_speeds(um, v, v2);
 #pragma unroll
@@ -1497,11 +1505,11 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
                 Jp[s] = v2[s];
             }
             __syncthreads();
-            stage_two_rows_regs<LC>(lds, j, q0);
-            stage_two_rows_regs<LC>(lds2, j, q1);
+            stage_two_rows_regs<LC, KEPT>(lds, j, q0);
+            stage_two_rows_regs<LC, KEPT>(lds2, j, q1);
             __syncthreads();
-            lds_read_c<LC>(v, lds, j);
-            lds_read_c<LC>(v2, lds2, j);
+            lds_read_any<LC, KEPT>(v, lds, j);
+            lds_read_any<LC, KEPT>(v2, lds2, j);
             line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // omega_x, omega_y
 #pragma unroll
             for (int s = 0; s < P; ++s)
