@@ -1344,17 +1344,19 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     // Staggered start.  The dispatcher gives every compute unit one workgroup
     // before it gives any a second (the host checks this on the device when a
     // context is created and passes an empty range otherwise), so blocks
-    // [ncu, 2 ncu) of member 0 are the second residents; they start about 16k
-    // cycles late.  The two workgroups
-    // of a CU then reach their gather phases at different times, each loading
-    // under the other's transforms, and keep that offset for the whole launch
-    // (a freed slot is refilled at once).  At 4096 (FP32, two resident
-    // workgroups per CU) this takes a launch from 93.1 to 91.0 us; half or
-    // 1.5 times the delay gains less.
+    // [ncu, 2 ncu) of member 0 are the second residents; they start about 12k
+    // cycles late.  The two workgroups of a CU then reach their gather phases
+    // at different times, each loading under the other's transforms, and keep
+    // that offset for the whole launch (a freed slot is refilled at once).  At
+    // 4096 (FP32, two resident workgroups per CU, wave-split transforms) this
+    // takes a launch from 72.9 to 71.8 us; 16k cycles (the best delay for the
+    // Stockham form, which waited at 27 barriers) measures 72.9 again and 24k
+    // cycles 75.6.
     if constexpr (MODE == ROW_STAGE && LC == 4096 && sizeof(T) == 4) {
         if (blockIdx.y == 0 && (int)blockIdx.x >= a.stagger_lo && (int)blockIdx.x < a.stagger_hi) {
-            __builtin_amdgcn_s_sleep(127);
-            __builtin_amdgcn_s_sleep(127);
+            __builtin_amdgcn_s_sleep(63);
+            __builtin_amdgcn_s_sleep(63);
+            __builtin_amdgcn_s_sleep(63);
         }
     }
     cx<T> v[P];
