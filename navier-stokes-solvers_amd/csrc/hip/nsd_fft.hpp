@@ -593,8 +593,8 @@ __device__ __forceinline__ void line_fft2_c(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T
 // waves and eight 512-point DFTs, one per wave:
 //   X[k1 + 8 k2] = sum_n2 W512^(n2 k2) W4096^(n2 k1) sum_n1 x[n2 + 512 n1] W8^(n1 k1).
 // Only the step between the two crosses waves.  The 512-point transform of
-// wave w runs on row w of the line region (512 elements, 544 with the
-// one-in-sixteen padding) in three radix-8 Stockham passes.  Its exchanges
+// wave w runs on row w of the line region (512 elements, padded one in
+// sixteen) in three radix-8 Stockham passes.  Its exchanges
 // stay inside the wave, whose LDS operations complete in order, so they need
 // a compiler fence and no workgroup barrier: two barriers per transform
 // remain instead of seven, and the cross-wave step reads and writes rows and
@@ -607,7 +607,14 @@ __device__ __forceinline__ void line_fft2_c(cx<T> (&v1)[P], cx<T> (&v2)[P], cx<T
 // A pointwise product between ws_a and ws_b transforms does not see the
 // pattern, which is how the row stage uses them.
 
-constexpr int WS_ROW = 544;  // padded row of 512 elements
+// Row w of a line region starts at 576 w: 512 elements with the
+// one-in-sixteen padding take 544, and a stride of 576 elements (4608 bytes
+// at FP32, a multiple of 512) lets the compiler pair a thread's accesses to
+// two rows into one ds_read2st64 / ds_write2st64.  Single 64-bit reads of
+// consecutive padded elements lose a cycle to a bank conflict wherever a
+// 32-lane phase crosses a padding gap; the paired forms do not.
+constexpr int WS_ROW = 576;
+constexpr int WS_REGION = 8 * WS_ROW;  // elements of a wave-split line region
 
 template <typename T>
 struct TwWs {

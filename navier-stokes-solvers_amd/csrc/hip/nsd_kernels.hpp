@@ -400,11 +400,7 @@ template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
 constexpr bool ws_lines64() {
     return sizeof(T) == 8 && LC == 4096 && NT == 512 && !paired_lines<T, LC, NT, PR>();
 }
-// line regions of the stage kernels
-template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
-constexpr int stage_regions() {
-    return ws_lines64<T, LC, NT, PR>() ? 2 : line_regions<T, LC, NT, PR>();
-}
+
 
 constexpr int slot_kind(int s, int L, int km) {
     const int tpl = L / P, lo = s * tpl, hi = lo + tpl - 1;
@@ -519,6 +515,23 @@ constexpr bool pair_layout() {
     return sizeof(T) == 4 && LC == 4096 && paired_lines<T, LC, NT, PR>();
 }
 
+// line regions of the stage kernels
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
+constexpr int stage_regions() {
+    return ws_lines64<T, LC, NT, PR>() ? 2 : line_regions<T, LC, NT, PR>();
+}
+// stage kernels that run wave-split transforms (their line regions hold
+// WS_REGION elements; staged lines in them are unpadded)
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
+constexpr bool ws_stage() {
+    return pair_layout<T, LC, NT, PR>() || ws_lines64<T, LC, NT, PR>();
+}
+// elements of one line region of a stage kernel (compile-time lengths)
+template <typename T, int LC, int NT = threads_for(LC), int PR = -1>
+constexpr int region_elems() {
+    return ws_stage<T, LC, NT, PR>() ? WS_REGION : LC + LC / 16;
+}
+
 // store (first, second) at pair element e of pair buffer B (one 16 bytes)
 __device__ __forceinline__ void store_pair(cx<float>* B, size_t e, cx<float> first,
                                            cx<float> second) {
@@ -582,8 +595,9 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
     constexpr int NB = stage_regions<T, LC, NT, PR>();
-    cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
-    cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
+    const int RE = LC > 0 ? region_elems<T, LC, NT, PR>() : Ll + Ll / 16;
+    cx<T>* lds = smem + m * NB * RE;
+    cx<T>* lds2 = lds + (NB - 1) * RE;
     constexpr bool WS = ws_lines64<T, LC, NT, PR>();
     // pair-layout kernels read J in 2 x 2 blocks: columns k and k ^ 1 share lines
     const int kxi = (pair_layout<T, LC, NT, PR>() ? xcd_pair_remap(blockIdx.x, gridDim.x)
@@ -1161,14 +1175,18 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
     const int Ll = LC > 0 ? LC : g.L;
     const int kxm = LC > 0 ? Band<LC>::kxm : G.kxmax;
     fft_any<-1, LC>(v, lds, j, g, tw);
+    // after a wave-split transform the line goes unpadded (only unit-stride
+    // accesses follow)
+    constexpr bool PADL = !std::is_same<TW, TwWs<T>>::value;
+    auto at = [](int i) { return PADL ? pad(i) : i; };
     if constexpr (LC > 0) {
         // the separation below reads kept columns k and their partners
         // Ll - k only, never a slot inside the dealiased band (kind 0)
         if constexpr (LC / P % 16 == 0) {
-            cx<T>* b = lds + pad(j);
+            cx<T>* b = lds + at(j);
 #pragma unroll
             for (int s = 0; s < P; ++s)
-                if (slot_kind(s, LC, Band<LC>::kxm) != 0) b[padc(s * TPL)] = v[s];
+                if (slot_kind(s, LC, Band<LC>::kxm) != 0) b[PADL ? padc(s * TPL) : s * TPL] = v[s];
         } else {
             lds_write_c<LC>(v, lds, j);
         }
@@ -1182,8 +1200,8 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
         if (kind == 0 || kind == 2) continue;
         const int k = j + s * TPL;
         if ((kind == 1 || k <= kxm) && active) {
-            const cx<T> zk = lds[pad(k)];
-            const cx<T> zm = lds[pad((Ll - k) & (Ll - 1))];
+            const cx<T> zk = lds[at(k)];
+            const cx<T> zm = lds[at((Ll - k) & (Ll - 1))];
             const cx<T> ja = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
             const cx<T> jb = mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
             if constexpr (JB) {
@@ -1204,14 +1222,15 @@ __device__ __forceinline__ void fwd_rows_store(cx<T> (&v)[P], cx<T>* lds, int j,
 // Standard-pattern read of a line staged with SKIP0: its kind-0 slots are
 // zero in registers, known to the compiler, so the first butterflies fold
 // them away, and they cost no LDS read.
-template <int LC, typename T>
+template <int LC, bool PAD = true, typename T>
 __device__ __forceinline__ void lds_read_kept(cx<T> (&v)[P], const cx<T>* lds, int j) {
     constexpr int TPL = LC / P;
     static_assert(TPL % 16 == 0, "compile-time offsets need TPL % 16 == 0");
-    const cx<T>* b = lds + pad(j);
+    const cx<T>* b = lds + (PAD ? pad(j) : j);
 #pragma unroll
     for (int s = 0; s < P; ++s)
-        v[s] = slot_kind(s, LC, Band<LC>::kxm) == 0 ? mk(T(0), T(0)) : b[padc(s * TPL)];
+        v[s] = slot_kind(s, LC, Band<LC>::kxm) == 0 ? mk(T(0), T(0))
+                                                    : b[PAD ? padc(s * TPL) : s * TPL];
 }
 
 template <int LC, bool KEPT, typename T>
@@ -1278,17 +1297,21 @@ __device__ __forceinline__ void gather_pair_rows(RowPairRegs<float>& f0, RowPair
 // The LDS image of stage_two_rows, from gathered registers.  SKIP0: the
 // slots that lie inside the dealiased band for every thread (kind 0) are
 // left unwritten, for a reader that takes them as zero (lds_read_kept).
-template <int LC, bool SKIP0 = false, typename T>
+// PAD: the line's one-in-sixteen padding, which the Stockham passes need;
+// a line read back only in the standard pattern goes unpadded (every access
+// to it is at unit stride).
+template <int LC, bool SKIP0 = false, bool PAD = true, typename T>
 __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const RowPairRegs<T>& r) {
     constexpr int TPL = LC / P, kxm = Band<LC>::kxm;
     const cx<T> zero = mk(T(0), T(0));
+    auto at = [](int i) { return PAD ? pad(i) : i; };
 #pragma unroll
     for (int s = 0; s < P; ++s) {
         const int kind = slot_kind(s, LC, kxm);
         const int k = j + s * TPL;
         if (kind == 2) continue;  // written by the owner of LC - k
         if (kind == 0) {
-            if (!SKIP0) lds[pad(k)] = zero;
+            if (!SKIP0) lds[at(k)] = zero;
             continue;
         }
         if (kind == 1 || k <= kxm) {
@@ -1297,10 +1320,10 @@ __device__ __forceinline__ void stage_two_rows_regs(cx<T>* lds, int j, const Row
                 a.y = T(0);
                 b.y = T(0);
             }
-            lds[pad(k)] = mk(a.x - b.y, a.y + b.x);
-            if (k > 0) lds[pad(LC - k)] = mk(a.x + b.y, b.x - a.y);
+            lds[at(k)] = mk(a.x - b.y, a.y + b.x);
+            if (k > 0) lds[at(LC - k)] = mk(a.x + b.y, b.x - a.y);
         } else if (k < LC - kxm) {
-            lds[pad(k)] = zero;
+            lds[at(k)] = zero;
         }
     }
 }
@@ -1337,8 +1360,9 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
     const int lines = NT / TPL;
     const int m = threadIdx.x / TPL, j = threadIdx.x - m * TPL;
     constexpr int NB = stage_regions<T, LC, NT, PR>();
-    cx<T>* lds = smem + m * NB * (Ll + Ll / 16);
-    cx<T>* lds2 = lds + (NB - 1) * (Ll + Ll / 16);
+    const int RE = LC > 0 ? region_elems<T, LC, NT, PR>() : Ll + Ll / 16;
+    cx<T>* lds = smem + m * NB * RE;
+    cx<T>* lds2 = lds + (NB - 1) * RE;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int item = blk * lines + m;  // row pair (STAGE, R2C) or row (C2R)
     // Staggered start.  The dispatcher gives every compute unit one workgroup
@@ -1427,8 +1451,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
                 gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
                 gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
             }
-            stage_two_rows_regs<LC, true>(lds, j, r0);
-            stage_two_rows_regs<LC, true>(lds2, j, r1);
+            stage_two_rows_regs<LC, true, false>(lds, j, r0);
+            stage_two_rows_regs<LC, true, false>(lds2, j, r1);
             if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
             else {
                 gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
@@ -1438,8 +1462,8 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             cx<T> v2[P];
             cx<T>(*const vv[2])[P] = {&v, &v2};
             cx<T>* const ll[2] = {lds, lds2};
-            lds_read_kept<LC>(v, lds, j);
-            lds_read_kept<LC>(v2, lds2, j);
+            lds_read_kept<LC, false>(v, lds, j);
+            lds_read_kept<LC, false>(v2, lds2, j);
             __syncthreads();
             ws_a<+1, 2>(vv, ll, j, tw);  // psi_x, psi_y
             if (a.track_umax) add_speeds(um, v, v2);
@@ -1449,11 +1473,11 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
                 Jp[s] = v2[s];
             }
             __syncthreads();
-            stage_two_rows_regs<LC, true>(lds, j, q0);
-            stage_two_rows_regs<LC, true>(lds2, j, q1);
+            stage_two_rows_regs<LC, true, false>(lds, j, q0);
+            stage_two_rows_regs<LC, true, false>(lds2, j, q1);
             __syncthreads();
-            lds_read_kept<LC>(v, lds, j);
-            lds_read_kept<LC>(v2, lds2, j);
+            lds_read_kept<LC, false>(v, lds, j);
+            lds_read_kept<LC, false>(v2, lds2, j);
             __syncthreads();
             ws_a<+1, 2>(vv, ll, j, tw);  // omega_x, omega_y
 #pragma unroll
@@ -1556,34 +1580,34 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             cx<T>* const rr[1] = {lds2};
             RowPairRegs<T> r;
             gather_two_rows<LC>(r, j, G, a.mixed[0] + moff, y0, active);
-            stage_two_rows_regs<LC, true>(lds, j, r);
+            stage_two_rows_regs<LC, true, false>(lds, j, r);
             gather_two_rows<LC>(r, j, G, a.mixed[1] + moff, y0, active);
             __syncthreads();
-            lds_read_kept<LC>(v, lds, j);
+            lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // psi_x
 #pragma unroll
             for (int s = 0; s < P; ++s) X[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
-            stage_two_rows_regs<LC, true>(lds, j, r);
+            stage_two_rows_regs<LC, true, false>(lds, j, r);
             gather_two_rows<LC>(r, j, G, a.mixed[2] + moff, y0, active);
             __syncthreads();
-            lds_read_kept<LC>(v, lds, j);
+            lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // psi_y
             if (a.track_umax) add_speeds(um, X, v);
 #pragma unroll
             for (int s = 0; s < P; ++s) Jp[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
-            stage_two_rows_regs<LC, true>(lds, j, r);
+            stage_two_rows_regs<LC, true, false>(lds, j, r);
             gather_two_rows<LC>(r, j, G, a.mixed[3] + moff, y0, active);
             __syncthreads();
-            lds_read_kept<LC>(v, lds, j);
+            lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // omega_x
 #pragma unroll
             for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
             __builtin_amdgcn_sched_barrier(0);
-            stage_two_rows_regs<LC, true>(lds, j, r);
+            stage_two_rows_regs<LC, true, false>(lds, j, r);
             __syncthreads();
-            lds_read_kept<LC>(v, lds, j);
+            lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1>(vv, rr, j, tw);  // omega_y
 #pragma unroll
             for (int s = 0; s < P; ++s)

@@ -41,7 +41,8 @@ size_t stage_lds() {
     constexpr int NT = var_nt<L, V>();
     constexpr int lines = NT * P / L;
     constexpr int PR = var_pr<V>();
-    return (size_t)lines * stage_regions<T, L, NT, PR>() * (L + L / 16) * sizeof(cx<T>);
+    return (size_t)lines * stage_regions<T, L, NT, PR>() * region_elems<T, L, NT, PR>() *
+           sizeof(cx<T>);
 }
 
 // ERR: the error controller's instantiations (launch variant 0 only)
