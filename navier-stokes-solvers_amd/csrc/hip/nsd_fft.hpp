@@ -670,42 +670,68 @@ __device__ __forceinline__ void wave_lds_fence() {
 // a and a + 2 sit 16 apart and the phase covers the 32 banks once.
 __device__ __forceinline__ int ws_lane8(int l) { return l ^ ((((l >> 3) ^ (l >> 4)) & 1) * 0x18); }
 
+// Positions inside a wave-split row.  Padded rows (SWZ false) keep the
+// one-in-sixteen padding: it spreads the stride-8 writes of the local passes
+// of 8-byte elements, and the compiler pairs their unit-stride reads into
+// ds_read2 forms, which padding gaps do not slow.  Swizzled rows (SWZ, for
+// 16-byte elements) are unpadded, element c at c ^ ((c >> 3) & 7): a 128-bit
+// write serves eight lanes per cycle on element banks mod 8, so the stride-8
+// writes need c mod 8 spread, and a unit-stride 128-bit read of padded
+// elements is slowed by the gaps (probe: twice the LDS cycles), while the
+// swizzle only permutes within aligned groups of eight.  It costs a few
+// registers for per-lane addresses, which the FP64 row stage (at its
+// register limit) cannot spare, so only the FP64 column stage uses it.
+// Either layout keeps the 64 s offsets of a lane's reads compile-time:
+// ws_pos(c + 64 s) = ws_pos(c) + ws_off(s).
+template <bool SWZ>
+__device__ __forceinline__ int ws_pos(int c) {
+    if constexpr (!SWZ) return padc(c);
+    else return c ^ ((c >> 3) & 7);
+}
+template <bool SWZ>
+constexpr int ws_off(int s) { return SWZ ? 64 * s : 68 * s; }
+
 // NF rows (1 or 2) of the wave's 512-point transform, lane l holding elements
 // l + 64 s of each row in and out.  row[f] is this wave's row of field f; its
 // contents may be overwritten.
-template <int DIR, int NF, typename T>
+template <int DIR, int NF, bool SWZ = false, typename T>
 __device__ __forceinline__ void ws_local(cx<T> (*const (&v)[NF])[P], cx<T>* const (&row)[NF],
                                          int l, const TwWs<T>& tw) {
-    // pass 0 (stride 1, no twiddles): butterfly l writes 8 l + r, which stays
-    // inside one 16-element padding group
+    static_assert(!SWZ || sizeof(T) == 8, "swizzled rows are measured for 16-byte elements");
+    constexpr bool F32 = !SWZ;  // padded rows
+    // pass 0 (stride 1, no twiddles): butterfly l writes 8 l + r (padded: one
+    // 16-element padding group; swizzled: r ^ (l mod 8) inside the lane's block)
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         dft8v<DIR>(*v[f]);
-        cx<T>* o = row[f] + padc(8 * l);
+        cx<T>* o = row[f] + (F32 ? padc(8 * l) : 8 * l);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) o[r] = (*v[f])[r];
+        for (int r = 0; r < 8; ++r) o[F32 ? r : (r ^ (l & 7))] = (*v[f])[r];
     }
     wave_lds_fence();
-    // pass 1 (stride 8)
+    // pass 1 (stride 8); padded rows swap bits 3 and 4 of the lane (ws_lane8)
     {
-        const int b = ws_lane8(l);
+        const int b = F32 ? ws_lane8(l) : l;
         const int k = b & 7;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            const cx<T>* i = row[f] + padc(b);
+            const cx<T>* i = row[f] + ws_pos<SWZ>(b);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) (*v[f])[s] = i[padc(64 * s)];
+            for (int s = 0; s < 8; ++s) (*v[f])[s] = i[ws_off<SWZ>(s)];
         }
         wave_lds_fence();
         cx<T> w[8];
         tw_powers8(w, tw.w8);
-        const int ob = padc((b - k) * 8) + k;  // padc(64 a + k) = 68 a + k
+        // outputs at 64 a + k + 8 r: padded 68 a + k + 8 r + r / 2,
+        // swizzled 64 a + 8 r + (k ^ r)
+        const int ob = F32 ? padc((b - k) * 8) + k : (b - k) * 8;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             tw_apply8<DIR>(*v[f], w);
             dft8v<DIR>(*v[f]);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) row[f][ob + 8 * r + (r >> 1)] = (*v[f])[r];
+            for (int r = 0; r < 8; ++r)
+                row[f][F32 ? ob + 8 * r + (r >> 1) : ob + 8 * r + (k ^ r)] = (*v[f])[r];
         }
     }
     wave_lds_fence();
@@ -713,9 +739,9 @@ __device__ __forceinline__ void ws_local(cx<T> (*const (&v)[NF])[P], cx<T>* cons
     {
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            const cx<T>* i = row[f] + padc(l);
+            const cx<T>* i = row[f] + ws_pos<SWZ>(l);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) (*v[f])[s] = i[padc(64 * s)];
+            for (int s = 0; s < 8; ++s) (*v[f])[s] = i[ws_off<SWZ>(s)];
         }
         wave_lds_fence();
         cx<T> w[8];
@@ -732,7 +758,7 @@ __device__ __forceinline__ void ws_local(cx<T> (*const (&v)[NF])[P], cx<T>* cons
 // wave-strided pattern out.  Every thread must be free to write the regions
 // on entry (the caller's barrier).  On return each wave still works in its
 // own rows: other waves may write the regions only after a workgroup barrier.
-template <int DIR, int NF, typename T>
+template <int DIR, int NF, bool SWZ = false, typename T>
 __device__ __forceinline__ void ws_a(cx<T> (*const (&v)[NF])[P], cx<T>* const (&lds)[NF], int j,
                                      const TwWs<T>& tw) {
     cx<T> w[8];
@@ -741,7 +767,7 @@ __device__ __forceinline__ void ws_a(cx<T> (*const (&v)[NF])[P], cx<T>* const (&
     for (int f = 0; f < NF; ++f) {
         dft8v<DIR>(*v[f]);
         tw_apply8<DIR>(*v[f], w);
-        cx<T>* o = lds[f] + padc(j);  // u[k1] of column j to row k1
+        cx<T>* o = lds[f] + ws_pos<SWZ>(j);  // u[k1] of column j to row k1
 #pragma unroll
         for (int r = 0; r < 8; ++r) o[r * WS_ROW] = (*v[f])[r];
     }
@@ -751,38 +777,38 @@ __device__ __forceinline__ void ws_a(cx<T> (*const (&v)[NF])[P], cx<T>* const (&
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         row[f] = lds[f] + wv * WS_ROW;
-        const cx<T>* i = row[f] + padc(l);
+        const cx<T>* i = row[f] + ws_pos<SWZ>(l);
 #pragma unroll
-        for (int s = 0; s < 8; ++s) (*v[f])[s] = i[padc(64 * s)];
+        for (int s = 0; s < 8; ++s) (*v[f])[s] = i[ws_off<SWZ>(s)];
     }
     wave_lds_fence();
-    ws_local<DIR, NF>(v, row, l, tw);
+    ws_local<DIR, NF, SWZ>(v, row, l, tw);
 }
 
 // NF 4096-point transforms, wave-strided pattern in, standard pattern out.
 // On entry only the wave's own rows of the regions have to be free (as they
 // are right after ws_a on the same regions); on return other threads may
 // still be reading the regions.
-template <int DIR, int NF, typename T>
+template <int DIR, int NF, bool SWZ = false, typename T>
 __device__ __forceinline__ void ws_b(cx<T> (*const (&v)[NF])[P], cx<T>* const (&lds)[NF], int j,
                                      const TwWs<T>& tw) {
     const int wv = j >> 6, l = j & 63;
     cx<T>* row[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) row[f] = lds[f] + wv * WS_ROW;
-    ws_local<DIR, NF>(v, row, l, tw);
+    ws_local<DIR, NF, SWZ>(v, row, l, tw);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-        cx<T>* o = row[f] + padc(l);  // Y_w[l + 64 s] to row w
+        cx<T>* o = row[f] + ws_pos<SWZ>(l);  // Y_w[l + 64 s] to row w
 #pragma unroll
-        for (int s = 0; s < 8; ++s) o[padc(64 * s)] = (*v[f])[s];
+        for (int s = 0; s < 8; ++s) o[ws_off<SWZ>(s)] = (*v[f])[s];
     }
     __syncthreads();
     cx<T> w[8];
     tw_powers8(w, tw.wj);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-        const cx<T>* i = lds[f] + padc(j);  // column j of every row
+        const cx<T>* i = lds[f] + ws_pos<SWZ>(j);  // column j of every row
 #pragma unroll
         for (int r = 0; r < 8; ++r) (*v[f])[r] = i[r * WS_ROW];
         tw_apply8<DIR>(*v[f], w);
@@ -822,10 +848,10 @@ __device__ __forceinline__ void ws_transpose(cx<T> (*const (&v)[NF])[P], cx<T>* 
 // same regions: each wave writes only its own row there).  Entry condition
 // as for ws_a on rows and, when tr differs, every thread free to write tr;
 // on return other threads may still read tr and work in their rows.
-template <int DIR, int NF, typename T>
+template <int DIR, int NF, bool SWZ = false, typename T>
 __device__ __forceinline__ void ws_c(cx<T> (*const (&v)[NF])[P], cx<T>* const (&rows)[NF],
                                      cx<T>* const (&tr)[NF], int j, const TwWs<T>& tw) {
-    ws_a<DIR, NF>(v, rows, j, tw);
+    ws_a<DIR, NF, SWZ>(v, rows, j, tw);
     ws_transpose<NF>(v, tr, j);
 }
 

@@ -655,7 +655,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             cx<T>(*const vv[1])[P] = {&v};
             cx<T>* const r1[1] = {lds};
             cx<T>* const t1[1] = {lds2};
-            ws_c<-1, 1>(vv, r1, t1, j, tw);
+            ws_c<-1, 1, true>(vv, r1, t1, j, tw);
         } else {
             fft_any<-1, LC>(v, lds, j, g, tw);
         }
@@ -973,8 +973,8 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
         }
     } else if constexpr (WS) {
         // FP64 columns of 4096: the four outputs one at a time through ws_c
-        // (spectral_core.c:62-99), rows in lds, the transposing exchange in
-        // lds2: two barriers per transform
+        // (spectral_core.c:62-99), swizzled rows in lds, the transposing
+        // exchange in lds2: two barriers per transform
         TwWs<T> tw;
         load_tw_ws(tw, j, a.tw);
         cx<T>(*const vv[1])[P] = {&v};
@@ -995,7 +995,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                 const T ky = T(col_band<LC>(G, j + s * TPL)) * kys;
                 v[s] = mul_i(w[s], (f & 1) ? ky : kx);
             }
-            ws_c<+1, 1>(vv, r1, t1, j, tw);
+            ws_c<+1, 1, true>(vv, r1, t1, j, tw);
             if (has_data) {
                 cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
