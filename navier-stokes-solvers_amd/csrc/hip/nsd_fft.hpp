@@ -670,26 +670,27 @@ __device__ __forceinline__ void wave_lds_fence() {
 // a and a + 2 sit 16 apart and the phase covers the 32 banks once.
 __device__ __forceinline__ int ws_lane8(int l) { return l ^ ((((l >> 3) ^ (l >> 4)) & 1) * 0x18); }
 
-// Positions inside a wave-split row.  Padded rows (SWZ false) keep the
-// one-in-sixteen padding: it spreads the stride-8 writes of the local passes
-// of 8-byte elements, and the compiler pairs their unit-stride reads into
-// ds_read2 forms, which padding gaps do not slow.  Swizzled rows (SWZ, for
-// 16-byte elements) are unpadded, element c at c ^ ((c >> 3) & 7): a 128-bit
-// write serves eight lanes per cycle on element banks mod 8, so the stride-8
-// writes need c mod 8 spread, and a unit-stride 128-bit read of padded
-// elements is slowed by the gaps (probe: twice the LDS cycles), while the
-// swizzle only permutes within aligned groups of eight.  It costs a few
-// registers for per-lane addresses, which the FP64 row stage (at its
-// register limit) cannot spare, so only the FP64 column stage uses it.
-// Either layout keeps the 64 s offsets of a lane's reads compile-time:
-// ws_pos(c + 64 s) = ws_pos(c) + ws_off(s).
+// Positions inside a wave-split row.  Each exchange (a write phase and the
+// read phase after it) may lay the row out its own way.
+// - Padded rows (SWZ false, 8-byte elements): every exchange keeps the
+//   one-in-sixteen padding.  It spreads the stride-8 writes of the local
+//   passes, and the compiler pairs the unit-stride reads into ds_read2 forms,
+//   which padding gaps do not slow.
+// - 16-byte rows (SWZ, single 128-bit accesses): a write serves eight lanes
+//   per cycle on element banks mod 8, and a unit-stride read of padded
+//   elements is slowed by the gaps (probe: twice the LDS cycles).  The
+//   exchanges at unit stride on both sides (cross-wave step, pass 1 to 2 with
+//   writes 64 a + k + 8 r, k = lane mod 8) go unpadded; pass 0 to 1, whose
+//   writes 8 l + r are stride 8, puts element c at c ^ ((c >> 3) & 7), which
+//   spreads them mod 8 and only permutes aligned groups of eight for the
+//   reads.
+// ws_pos(c + 64 s) = ws_pos(c) + ws_off(s) in every layout, so a lane's
+// reads take compile-time offsets.
 template <bool SWZ>
-__device__ __forceinline__ int ws_pos(int c) {
-    if constexpr (!SWZ) return padc(c);
-    else return c ^ ((c >> 3) & 7);
-}
+__device__ __forceinline__ int ws_pos(int c) { return SWZ ? c : padc(c); }
 template <bool SWZ>
 constexpr int ws_off(int s) { return SWZ ? 64 * s : 68 * s; }
+__device__ __forceinline__ int ws_swz(int c) { return c ^ ((c >> 3) & 7); }
 
 // NF rows (1 or 2) of the wave's 512-point transform, lane l holding elements
 // l + 64 s of each row in and out.  row[f] is this wave's row of field f; its
@@ -697,41 +698,39 @@ constexpr int ws_off(int s) { return SWZ ? 64 * s : 68 * s; }
 template <int DIR, int NF, bool SWZ = false, typename T>
 __device__ __forceinline__ void ws_local(cx<T> (*const (&v)[NF])[P], cx<T>* const (&row)[NF],
                                          int l, const TwWs<T>& tw) {
-    static_assert(!SWZ || sizeof(T) == 8, "swizzled rows are measured for 16-byte elements");
-    constexpr bool F32 = !SWZ;  // padded rows
+    static_assert(!SWZ || sizeof(T) == 8, "16-byte row layouts");
     // pass 0 (stride 1, no twiddles): butterfly l writes 8 l + r (padded: one
-    // 16-element padding group; swizzled: r ^ (l mod 8) inside the lane's block)
+    // 16-element padding group; 16-byte rows: r ^ (l mod 8) inside the
+    // lane's block of eight)
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         dft8v<DIR>(*v[f]);
-        cx<T>* o = row[f] + (F32 ? padc(8 * l) : 8 * l);
+        cx<T>* o = row[f] + (SWZ ? 8 * l : padc(8 * l));
 #pragma unroll
-        for (int r = 0; r < 8; ++r) o[F32 ? r : (r ^ (l & 7))] = (*v[f])[r];
+        for (int r = 0; r < 8; ++r) o[SWZ ? (r ^ (l & 7)) : r] = (*v[f])[r];
     }
     wave_lds_fence();
     // pass 1 (stride 8); padded rows swap bits 3 and 4 of the lane (ws_lane8)
     {
-        const int b = F32 ? ws_lane8(l) : l;
+        const int b = SWZ ? l : ws_lane8(l);
         const int k = b & 7;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            const cx<T>* i = row[f] + ws_pos<SWZ>(b);
+            const cx<T>* i = row[f] + (SWZ ? ws_swz(b) : padc(b));
 #pragma unroll
             for (int s = 0; s < 8; ++s) (*v[f])[s] = i[ws_off<SWZ>(s)];
         }
         wave_lds_fence();
         cx<T> w[8];
         tw_powers8(w, tw.w8);
-        // outputs at 64 a + k + 8 r: padded 68 a + k + 8 r + r / 2,
-        // swizzled 64 a + 8 r + (k ^ r)
-        const int ob = F32 ? padc((b - k) * 8) + k : (b - k) * 8;
+        // outputs at 64 a + k + 8 r (padded: 68 a + k + 8 r + r / 2)
+        const int ob = SWZ ? (b - k) * 8 + k : padc((b - k) * 8) + k;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             tw_apply8<DIR>(*v[f], w);
             dft8v<DIR>(*v[f]);
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
-                row[f][F32 ? ob + 8 * r + (r >> 1) : ob + 8 * r + (k ^ r)] = (*v[f])[r];
+            for (int r = 0; r < 8; ++r) row[f][SWZ ? ob + 8 * r : ob + 8 * r + (r >> 1)] = (*v[f])[r];
         }
     }
     wave_lds_fence();

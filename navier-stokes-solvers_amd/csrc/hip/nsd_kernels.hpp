@@ -327,7 +327,7 @@ __device__ __forceinline__ void fft_any(cx<T> (&v)[P], cx<T>* lds, int j, const 
     static_assert(LC == 4096, "wave-split transforms are 4096-point");
     cx<T>(*const vv[1])[P] = {&v};
     cx<T>* const ll[1] = {lds};
-    ws_b<DIR, 1>(vv, ll, j, tw);
+    ws_b<DIR, 1, (sizeof(T) == 8)>(vv, ll, j, tw);  // 16-byte rows at FP64
     __syncthreads();
 }
 
@@ -1584,7 +1584,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             gather_two_rows<LC>(r, j, G, a.mixed[1] + moff, y0, active);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
-            ws_a<+1, 1>(vv, rr, j, tw);  // psi_x
+            ws_a<+1, 1, true>(vv, rr, j, tw);  // psi_x
 #pragma unroll
             for (int s = 0; s < P; ++s) X[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
@@ -1592,7 +1592,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             gather_two_rows<LC>(r, j, G, a.mixed[2] + moff, y0, active);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
-            ws_a<+1, 1>(vv, rr, j, tw);  // psi_y
+            ws_a<+1, 1, true>(vv, rr, j, tw);  // psi_y
             if (a.track_umax) add_speeds(um, X, v);
 #pragma unroll
             for (int s = 0; s < P; ++s) Jp[s] = v[s];
@@ -1601,14 +1601,14 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             gather_two_rows<LC>(r, j, G, a.mixed[3] + moff, y0, active);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
-            ws_a<+1, 1>(vv, rr, j, tw);  // omega_x
+            ws_a<+1, 1, true>(vv, rr, j, tw);  // omega_x
 #pragma unroll
             for (int s = 0; s < P; ++s) Jp[s] = mk(-Jp[s].x * v[s].x, -Jp[s].y * v[s].y);
             __builtin_amdgcn_sched_barrier(0);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
-            ws_a<+1, 1>(vv, rr, j, tw);  // omega_y
+            ws_a<+1, 1, true>(vv, rr, j, tw);  // omega_y
 #pragma unroll
             for (int s = 0; s < P; ++s)
                 v[s] = mk(Jp[s].x + X[s].x * v[s].x, Jp[s].y + X[s].y * v[s].y);
