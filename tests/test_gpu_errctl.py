@@ -113,6 +113,32 @@ def test_error_controller_fp64_matches_oracle(ns, oc, case):
     s.close()
 
 
+def test_error_controller_fp64_4096_matches_oracle(ns, oc):
+    """The 4096^2 stage kernels with the controller (their wave-split
+    transforms, the embedded estimate, the judge and the restoring step)
+    against the oracle.  At rtol 1e-8 the first candidate's error is about
+    1.6: it is rejected, the step restored and retried with a smaller dt, and
+    the next candidates are accepted.  The estimate |y4 - y2| is then ~1e-8
+    of |y|, so the judged errors agree to the FP64 round-off of that
+    difference (1e-6 relative), far from the threshold at 1."""
+    n, attempts, rtol = 4096, 3, 1e-8
+    o, ctl = oracle_run(oc, "if_forced", n, attempts, rtol)
+    s = gpu_run(ns, "if_forced", n, attempts, rtol)
+    acc, rej, err, dt_next = s.error_control_stats()
+    w_err = rel_l2(s.spec("omega_hat"), o.spec("omega_hat"))
+    print(f"4096^2: GPU {acc} accepted / {rej} rejected, oracle {ctl.accepted} / {ctl.rejected}; "
+          f"t {s.d.t:.12e} vs {o.d.t:.12e}; last err {err:.6f} vs {ctl.err:.6f}; "
+          f"omega_hat rel L2 {w_err:.2e}", flush=True)
+    assert ctl.rejected >= 1 and ctl.accepted >= 1
+    assert (acc, rej) == (ctl.accepted, ctl.rejected)
+    assert s.d.t == pytest.approx(o.d.t, rel=1e-12)
+    assert s.d.dt == pytest.approx(o.d.dt, rel=1e-12)
+    assert err == pytest.approx(ctl.err, rel=1e-6)
+    assert dt_next == pytest.approx(ctl.dt_err, rel=1e-6)
+    assert w_err <= 1e-11
+    s.close()
+
+
 def test_error_controller_fp32_stages(ns, oc):
     n, attempts, rtol = 64, 60, 1e-6
     o, ctl = oracle_run(oc, "if_forced", n, attempts, rtol)
