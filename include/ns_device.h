@@ -142,7 +142,7 @@ int nsd_member_download(NsdContext *c, int member, double *omega_hat);
  * before setting the initial field.  Every variant gives identical results. */
 int nsd_autotune(NsdContext *c, int reps, int force, int *row_variant, int *col_variant);
 /* Staggered start of the 4096-point FP32 row stage: the second resident
- * workgroup of each compute unit starts about 7 us late, so that the two
+ * workgroup of each compute unit starts about 5 us late, so that the two
  * workgroups of a CU reach their gathers at different times.  It is on when a
  * probe launch at context creation shows that the device gives every CU one
  * workgroup before any gets a second (the dispatch order the delay assumes).
