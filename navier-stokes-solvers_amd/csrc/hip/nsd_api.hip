@@ -847,15 +847,20 @@ int lastk(NsdContext* c) {
     return ST_OK;
 }
 
-// The stage kernels' mixed fields use the pair layout (pair_layout in
-// nsd_kernels.hpp) when the row and the column instantiations this context
-// launches both hold it (square FP32 4096^2 grids with their paired
-// transforms); the answer follows the launch variants.
+// The stage kernels' mixed fields use a layout of their own (stage_layout in
+// nsd_kernels.hpp: the pair layout at FP32, row-pair tiles at FP64) when the
+// row and the column instantiations this context launches both hold it
+// (square 4096^2 grids); the answer follows the launch variants.
 bool pairs_ok(const NsdContext* c) {
-    return c->prec == NSD_FP32 && c->G.nx == c->G.ny &&
-           stage_variant_pairs<float>(c->G.nx, c->vrow) &&
-           stage_variant_pairs<float>(c->G.ny, c->vcol);
+    if (c->G.nx != c->G.ny) return false;
+    if (c->prec == NSD_FP32)
+        return stage_variant_pairs<float>(c->G.nx, c->vrow) &&
+               stage_variant_pairs<float>(c->G.ny, c->vcol);
+    return stage_variant_pairs<double>(c->G.nx, c->vrow) &&
+           stage_variant_pairs<double>(c->G.ny, c->vcol);
 }
+// J in 2 x 2 blocks: written by the FP32 pair-layout row stage
+bool jblocks_ok(const NsdContext* c) { return c->prec == NSD_FP32 && pairs_ok(c); }
 
 template <typename T>
 ColArgs<T> col_args(NsdContext* c) {
@@ -1314,7 +1319,7 @@ int op_nonlinear(NsdContext* c, const double* w_ref, const double* psi_ref, doub
     launch_row_t<T>(c, r, c->G.ny / 2);
     ColArgs<T> f = col_args<T>(c);
     f.mode = COL_FWD;
-    f.jblocks = pairs_ok(c) ? 1 : 0;  // J of a pair-layout row stage
+    f.jblocks = jblocks_ok(c) ? 1 : 0;  // J of a pair-layout row stage
     f.out_ref = out_ref;
     f.fwd_scale = -1.0;  // spectral_core.c:108-111
     launch_col_t<T>(c, f, c->G.nkx);
@@ -1379,7 +1384,7 @@ int op_nl_member(NsdContext* c, int m, double* out_ref, int compute_rows) {
     ColArgs<T> f = col_args<T>(c);
     f.mode = COL_FWD;
     f.nl = (const cx<T>*)c->nl + (size_t)m * M_of(c);
-    f.jblocks = pairs_ok(c) ? 1 : 0;  // J of a pair-layout row stage
+    f.jblocks = jblocks_ok(c) ? 1 : 0;  // J of a pair-layout row stage
     f.out_ref = out_ref;
     f.fwd_scale = -1.0;  // spectral_core.c:108-111
     launch_col_t<T>(c, f, c->G.nkx);

@@ -863,15 +863,18 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-// Column pairs on one XCD: blocks b and b + 8 (the same XCD, one after the
-// other in its dispatch order) get columns 2 p and 2 p + 1, pair p = 8 (b /
-// 16) + b % 8, so that two columns that share cache lines are read through
-// one L2 at about the same time.  A bijection on [0, nb): the blocks past the
-// last multiple of 16 keep their index.
-__device__ __forceinline__ int xcd_pair_remap(int b, int nb) {
-    if (b >= (nb & ~15)) return b;
+// Column groups on one XCD: blocks b, b + 8, ..., b + 8 (G - 1) (the same
+// XCD, one after the other in its dispatch order) get columns G q .. G q +
+// G - 1, group q = 8 (b / 8 G) + b % 8, so that G columns that share cache
+// lines are read or written through one L2 at about the same time.  A
+// bijection on [0, nb): the blocks past the last multiple of 8 G keep their
+// index.
+template <int G>
+__device__ __forceinline__ int xcd_group_remap(int b, int nb) {
+    static_assert(G == 1 || G == 2 || G == 4, "groups of 1, 2 or 4 columns");
+    if (G == 1 || b >= (nb & ~(8 * G - 1))) return b;
     const int x = b & 7, idx = b >> 3;
-    return ((((idx >> 1) << 3) + x) << 1) + (idx & 1);
+    return ((idx / G) * 8 + x) * G + idx % G;
 }
 
 }  // namespace nsd

@@ -353,6 +353,38 @@ __device__ __forceinline__ unsigned jblock_elem(int k, int ny, int y) {
            ((unsigned)(k & 1) << 1) + (unsigned)(y & 1);
 }
 
+// Element (k, y) of a mixed field in row-pair tiles: ROWPAIR_TY rows of all K
+// buffer columns in one run, column by column, (y - y % TY) K + TY k + y % TY.
+// With TY = 2 (FP64, 32 bytes per column) a row pair's elements are one
+// contiguous run and ROWPAIR_CPL = 4 columns share a 128-byte line.
+constexpr int ROWPAIR_TY = 2;
+constexpr int ROWPAIR_CPL = 128 / (ROWPAIR_TY * 16);
+__device__ __forceinline__ unsigned rowpair_elem(int k, int K, int y) {
+    constexpr unsigned TY = ROWPAIR_TY;
+    return __umul24((unsigned)y & ~(TY - 1), (unsigned)K) + TY * (unsigned)k +
+           ((unsigned)y & (TY - 1));
+}
+
+// This thread's slots of one line of a mixed field: slot s at element
+// e0 + s es.
+struct SlotElems {
+    unsigned e0, es;
+};
+// Row stage, rows y, y + 1 (y even): slot s is column j + s LC/P.
+template <int LC>
+__device__ __forceinline__ SlotElems row_slots(int j, int ny, int K, int y, bool tiles) {
+    if (tiles) return {rowpair_elem(j, K, y), (unsigned)ROWPAIR_TY * (LC / P)};
+    return {col_elem(j, ny, y), (unsigned)ny * (LC / P)};
+}
+// Column stage, column k: slot s is row j + s LC/P (LC / P a multiple of
+// ROWPAIR_TY).
+template <int LC>
+__device__ __forceinline__ SlotElems col_slots(int k, int j, int ny, int K, bool tiles) {
+    static_assert((LC / P) % ROWPAIR_TY == 0, "slots step by whole tiles");
+    if (tiles) return {rowpair_elem(k, K, j), (unsigned)K * (LC / P)};
+    return {col_elem(k, ny, j), (unsigned)(LC / P)};
+}
+
 template <typename T>
 __device__ __forceinline__ const cx<T>* at32(const cx<T>* F, unsigned i) {
     return reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(F) +
@@ -441,7 +473,7 @@ struct ColArgs {
     const cx<T>* tw;  // twiddles for length ny
     const cx<T>* nl;  // mixed input (COL_STAGE, COL_FWD)
     cx<T>* mixed[4];  // mixed outputs
-    int pairs;        // COL_STAGE, COL_PRIME: write them in the pair layout (pair_layout)
+    int pairs;        // COL_STAGE, COL_PRIME: write them in the stage layout (stage_layout)
     int jblocks;      // COL_FWD: J in 2 x 2 blocks (written by a pair-layout row stage)
     cx<T>* state;     // compact state (read; written on the last stage)
     cx<T>* acc;       // compact RK accumulator
@@ -513,6 +545,28 @@ __device__ __forceinline__ cx<T> grow(cx<T> w, T em1) {
 template <typename T, int LC, int NT, int PR>
 constexpr bool pair_layout() {
     return sizeof(T) == 4 && LC == 4096 && paired_lines<T, LC, NT, PR>();
+}
+
+// Row-pair tiles of the mixed fields (rowpair_elem): the FP64 stage kernels
+// of 4096 (ws_lines64), selected per launch like the pair layout.  A row
+// pair's gathers of one field then read one contiguous run of 32 bytes per
+// kept column, neighbouring columns adjacent (2 KB per wave instruction),
+// where the per-field layout puts every lane in a 64 KB column of its own and
+// a row pair's gathers span the fields' whole 358 MB; k_col writes 32-byte
+// runs, ROWPAIR_CPL columns to a 128-byte line (xcd_group_remap keeps them on
+// one XCD, one after the other).  J stays [kx][y]: in tiles of 2, 4 or 8
+// rows the column stage's reads of it cost more than the row stage's stores
+// gained.
+template <typename T, int LC, int NT, int PR>
+constexpr bool rowpair_layout() {
+    return ws_lines64<T, LC, NT, PR>();
+}
+
+// the stage kernels' own layout of the mixed fields, if any (RowArgs and
+// ColArgs::pairs select it per launch)
+template <typename T, int LC, int NT, int PR>
+constexpr bool stage_layout() {
+    return pair_layout<T, LC, NT, PR>() || rowpair_layout<T, LC, NT, PR>();
 }
 
 // line regions of the stage kernels
@@ -599,9 +653,12 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     cx<T>* lds = smem + m * NB * RE;
     cx<T>* lds2 = lds + (NB - 1) * RE;
     constexpr bool WS = ws_lines64<T, LC, NT, PR>();
-    // pair-layout kernels read J in 2 x 2 blocks: columns k and k ^ 1 share lines
-    const int kxi = (pair_layout<T, LC, NT, PR>() ? xcd_pair_remap(blockIdx.x, gridDim.x)
-                                                   : (int)blockIdx.x) * lines + m;
+    // pair-layout kernels read J in 2 x 2 blocks: columns k and k ^ 1 share
+    // lines; row-pair tiles put ROWPAIR_CPL columns in one line
+    constexpr int XG = pair_layout<T, LC, NT, PR>()      ? 2
+                       : rowpair_layout<T, LC, NT, PR>() ? ROWPAIR_CPL
+                                                         : 1;
+    const int kxi = xcd_group_remap<XG>(blockIdx.x, gridDim.x) * lines + m;
     const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
     const bool active = kxi < ncols;
     const bool has_data = active && kxi < G.nkx_keep;
@@ -980,6 +1037,8 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
         cx<T>(*const vv[1])[P] = {&v};
         cx<T>* const r1[1] = {lds};
         cx<T>* const t1[1] = {lds2};
+        // the outputs in row-pair tiles or per field ([kx][y])
+        const SlotElems se = col_slots<LC>(kxi, j, G.ny, (G.nkx_keep + 1) & ~1, a.pairs != 0);
 #pragma unroll 1
         for (int f = 3; f >= 0; --f) {
             if (f == 1) {
@@ -999,7 +1058,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             if (has_data) {
                 cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
-                for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
+                for (int s = 0; s < P; ++s) *at32(out, se.e0 + (unsigned)s * se.es) = v[s];
             }
         }
     } else {
@@ -1061,7 +1120,7 @@ struct RowArgs {
     int stagger_hi;
     unsigned short* out_bf16;  // ROW_C2R: field 0 as bf16, times out_bf16_scale (if non-null)
     float out_bf16_scale;
-    int pairs;                 // ROW_STAGE: mixed fields in the pair layout (pair_layout)
+    int pairs;                 // ROW_STAGE: mixed fields in the stage layout (stage_layout)
     int skip_on_redo;          // ROW_STAGE, ERR: idle when the step restores a rejected one
 };
 
@@ -1258,6 +1317,29 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
         if (kind == 1 || (kind == 3 && k <= kxm)) {
             if (active) {
                 const cx<T>* src = at32(F, col_elem(k, G.ny, y));
+                r.a[s] = src[0];
+                r.b[s] = src[1];
+            } else {
+                r.a[s] = mk(T(0), T(0));
+                r.b[s] = mk(T(0), T(0));
+            }
+        }
+    }
+}
+
+// The same from slot elements (row_slots): the per-field layout or row-pair
+// tiles, rows y and y + 1 adjacent in both.
+template <int LC, typename T>
+__device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const cx<T>* F,
+                                                SlotElems se, bool active) {
+    constexpr int kxm = Band<LC>::kxm;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+        const int kind = slot_kind(s, LC, kxm);
+        const int k = j + s * (LC / P);
+        if (kind == 1 || (kind == 3 && k <= kxm)) {
+            if (active) {
+                const cx<T>* src = at32(F, se.e0 + (unsigned)s * se.es);
                 r.a[s] = src[0];
                 r.b[s] = src[1];
             } else {
@@ -1578,10 +1660,12 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
         if (blk * lines < G.ny / 2) {
             cx<T>(*const vv[1])[P] = {&v};
             cx<T>* const rr[1] = {lds2};
+            // the inputs in row-pair tiles or per field ([kx][y])
+            const SlotElems se = row_slots<LC>(j, G.ny, (G.nkx_keep + 1) & ~1, y0, a.pairs != 0);
             RowPairRegs<T> r;
-            gather_two_rows<LC>(r, j, G, a.mixed[0] + moff, y0, active);
+            gather_two_rows<LC>(r, j, a.mixed[0] + moff, se, active);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
-            gather_two_rows<LC>(r, j, G, a.mixed[1] + moff, y0, active);
+            gather_two_rows<LC>(r, j, a.mixed[1] + moff, se, active);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1, true>(vv, rr, j, tw);  // psi_x
@@ -1589,7 +1673,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             for (int s = 0; s < P; ++s) X[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
-            gather_two_rows<LC>(r, j, G, a.mixed[2] + moff, y0, active);
+            gather_two_rows<LC>(r, j, a.mixed[2] + moff, se, active);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1, true>(vv, rr, j, tw);  // psi_y
@@ -1598,7 +1682,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             for (int s = 0; s < P; ++s) Jp[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
-            gather_two_rows<LC>(r, j, G, a.mixed[3] + moff, y0, active);
+            gather_two_rows<LC>(r, j, a.mixed[3] + moff, se, active);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1, true>(vv, rr, j, tw);  // omega_x

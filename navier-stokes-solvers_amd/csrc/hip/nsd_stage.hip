@@ -108,7 +108,7 @@ void limits_LV(int bytes) {
 template <typename T, int LOG, int V>
 int var_pairs() {
     constexpr int L = 1 << LOG;
-    if constexpr (var_ok<L, V>()) return pair_layout<T, L, var_nt<L, V>(), var_pr<V>()>() ? 1 : 0;
+    if constexpr (var_ok<L, V>()) return stage_layout<T, L, var_nt<L, V>(), var_pr<V>()>() ? 1 : 0;
     return 0;
 }
 

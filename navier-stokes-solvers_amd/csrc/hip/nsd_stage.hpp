@@ -26,7 +26,7 @@ int col_fused_blocks(int L, int ncols);
 template <typename T>
 int stage_variant_valid(int L, int variant, int maxlds);
 // 1 if the stage kernels of `variant` at length L read (k_row) and write
-// (k_col) the pair layout of the mixed fields (pair_layout)
+// (k_col) the mixed fields in a layout of their own (stage_layout)
 template <typename T>
 int stage_variant_pairs(int L, int variant);
 // raise the dynamic-LDS limit of every instantiation above

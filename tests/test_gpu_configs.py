@@ -182,6 +182,38 @@ def test_pair_layout_transitions_4096_fp32(ns):
         s.close()
 
 
+def test_rowpair_tiles_transitions_4096_fp64(ns):
+    """At FP64 4096^2 the stage kernels hold the mixed fields in row-pair
+    tiles (rows y, y + 1 of every kept kx in one run); the compat transforms
+    write and read the per-field layout in the same buffers.  Two steps taken
+    after the velocity and nonlinear-term entry points (the prime pass
+    rewrites the tiles) equal, bit for bit, two steps taken without them, and
+    the nonlinear term itself matches the FP32 one to FP32 round-off."""
+    n = 4096
+    made = []
+    for fp32, compat in ((False, True), (False, False), (True, True)):
+        s = ns.Spectral(n, n, TWO_PI, TWO_PI, 1e4)
+        s.init(taylor_green=False)
+        s.set_precision(fp32)
+        s.random_phase(5, k_peak=10.0)
+        if compat:
+            s.L.ns_spectral_compute_velocity_from_vorticity(s.p)
+            s.L.ns_spectral_compute_nonlinear_terms(s.p)
+            s.check()
+        made.append(s)
+    tiled, plain, f32 = made
+    err = rel_l2(f32.spec("nonlinear_hat"), tiled.spec("nonlinear_hat"))
+    print(f"nonlinear term FP32 vs FP64 at 4096^2: rel L2 {err:.2e}", flush=True)
+    assert err < 1e-5
+    for s in (tiled, plain):
+        assert s.advance_n(2, True) == 0
+        assert s.sync_host() == 0
+    assert tiled.d.t == plain.d.t
+    assert np.array_equal(tiled.spec("omega_hat"), plain.spec("omega_hat"))
+    for s in made:
+        s.close()
+
+
 def test_row_stagger_dispatch_check_and_identical_steps(ns):
     """The 4096-point FP32 row stage delays the second resident workgroup of
     each compute unit.  A probe launch at context creation checks the
