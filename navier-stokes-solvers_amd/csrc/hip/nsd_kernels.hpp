@@ -693,6 +693,18 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
 #pragma unroll
             for (int s = 0; s < P; ++s)
                 v[s] = has_data ? *at32(nl, jb0 + 2u * (unsigned)(s * TPL)) : zero;
+        } else if constexpr (rowpair_layout<T, LC, NT, PR>()) {
+            // J's last reader, whole columns: non-temporal loads, which leave
+            // the caches to the mixed fields written next (FP64 4096^2:
+            // k_col 177.1 -> 174.0 us); every column of the grid is kept
+            // (COL_STAGE launches nkx_keep of them), so the loads are in bounds
+            typedef T t2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const t2 q = __builtin_nontemporal_load(
+                    reinterpret_cast<const t2*>(nl + mbase + j + s * TPL));
+                v[s] = has_data ? mk(q.x, q.y) : zero;
+            }
         } else {
 #pragma unroll
             for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
@@ -1328,10 +1340,17 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
 }
 
 // The same from slot elements (row_slots): the per-field layout or row-pair
-// tiles, rows y and y + 1 adjacent in both.
+// tiles, rows y and y + 1 adjacent in both.  Non-temporal loads: in row-pair
+// tiles no other row pair reads these bytes, and the stage reads each field
+// once, so they need not displace from L2 and the MALL the J values and
+// compact fields that k_col reads next (FP64 4096^2 step 1.368 -> 1.346 ms:
+// k_row +3.7 us, k_col -9.7 us).  The FP32 pair layout keeps ordinary loads:
+// neighbouring row pairs share its lines, and the same loads there took
+// k_row from 76 to 137 us.
 template <int LC, typename T>
 __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const cx<T>* F,
                                                 SlotElems se, bool active) {
+    typedef T t2 __attribute__((ext_vector_type(2)));
     constexpr int kxm = Band<LC>::kxm;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
@@ -1339,9 +1358,11 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
         const int k = j + s * (LC / P);
         if (kind == 1 || (kind == 3 && k <= kxm)) {
             if (active) {
-                const cx<T>* src = at32(F, se.e0 + (unsigned)s * se.es);
-                r.a[s] = src[0];
-                r.b[s] = src[1];
+                const t2* src = reinterpret_cast<const t2*>(at32(F, se.e0 + (unsigned)s * se.es));
+                const t2 a = __builtin_nontemporal_load(src);
+                const t2 b = __builtin_nontemporal_load(src + 1);
+                r.a[s] = mk(a.x, a.y);
+                r.b[s] = mk(b.x, b.y);
             } else {
                 r.a[s] = mk(T(0), T(0));
                 r.b[s] = mk(T(0), T(0));
