@@ -72,6 +72,27 @@ def test_members_match_standalone_runs(ns, fp32):
     e.close()
 
 
+def test_fp64_4096_members_match_standalone_runs(ns):
+    """Two FP64 4096^2 members in one batch: the stage kernels of this size
+    hold the mixed fields in row-pair tiles, and the second member's tiles,
+    J and compact fields sit one member stride further on.  Each member
+    follows its standalone run to FP64 round-off."""
+    sizes, Re, seeds, steps = [4096, 4096], [1e4, 3e3], [7, 8], 2
+    e = make_ensemble(ns, sizes, Re, seeds, False)
+    e.advance(steps)
+    t, E, Z, st = e.diagnostics()
+    assert np.all(st == 0)
+    for m, (n, r, sd) in enumerate(zip(sizes, Re, seeds)):
+        w = e.state(m)
+        w_ref, t_ref = single_run(ns, n, r, sd, steps, False, True)
+        err = rel_l2(w, w_ref)
+        print(f"member {m}: n={n} Re={r:g} seed={sd} t={t[m]:.6f} rel L2 vs standalone {err:.1e}",
+              flush=True)
+        assert t[m] == pytest.approx(t_ref, rel=1e-12)
+        assert err <= 1e-12
+    e.close()
+
+
 def test_fp32_members_vs_fp64(ns):
     rng = np.random.default_rng(7)
     sizes = list(rng.choice([64, 128, 256], size=12))
