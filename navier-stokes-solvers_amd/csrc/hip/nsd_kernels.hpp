@@ -1344,9 +1344,11 @@ __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const 
 // tiles no other row pair reads these bytes, and the stage reads each field
 // once, so they need not displace from L2 and the MALL the J values and
 // compact fields that k_col reads next (FP64 4096^2 step 1.368 -> 1.346 ms:
-// k_row +3.7 us, k_col -9.7 us).  The FP32 pair layout keeps ordinary loads:
-// neighbouring row pairs share its lines, and the same loads there took
-// k_row from 76 to 137 us.
+// k_row +3.7 us, k_col -9.7 us).  The per-field layout, which only the
+// nonlinear-term entry point hands this kernel, shares each line among four
+// row pairs and pays for the loads there; that call is not on the step's
+// path.  The FP32 pair layout keeps ordinary loads (gather_pair_rows): the
+// same loads there took k_row from 76 to 137 us.
 template <int LC, typename T>
 __device__ __forceinline__ void gather_two_rows(RowPairRegs<T>& r, int j, const cx<T>* F,
                                                 SlotElems se, bool active) {
