@@ -1549,19 +1549,23 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
         TwWs<T> tw;
         load_tw_ws(tw, j, a.tw);
         if (blk * lines < G.ny / 2) {
+            // one line per workgroup: inside this branch its row pair exists,
+            // and the gathers need no zero fill for an inactive one
+            static_assert(NT * P / LC == 1, "one row pair per workgroup");
+            constexpr bool live = true;
             RowPairRegs<T> r0, r1, q0, q1;
             const bool pl = a.pairs != 0;
-            if (pl) gather_pair_rows<LC>(r0, r1, j, G, a.mixed[0] + 2 * moff, y0, active);
+            if (pl) gather_pair_rows<LC>(r0, r1, j, G, a.mixed[0] + 2 * moff, y0, live);
             else {
-                gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, active);
-                gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, active);
+                gather_two_rows<LC>(r0, j, G, a.mixed[0] + moff, y0, live);
+                gather_two_rows<LC>(r1, j, G, a.mixed[1] + moff, y0, live);
             }
             stage_two_rows_regs<LC, true, false>(lds, j, r0);
             stage_two_rows_regs<LC, true, false>(lds2, j, r1);
-            if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, active);
+            if (pl) gather_pair_rows<LC>(q0, q1, j, G, a.mixed[2] + 2 * moff, y0, live);
             else {
-                gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, active);
-                gather_two_rows<LC>(q1, j, G, a.mixed[3] + moff, y0, active);
+                gather_two_rows<LC>(q0, j, G, a.mixed[2] + moff, y0, live);
+                gather_two_rows<LC>(q1, j, G, a.mixed[3] + moff, y0, live);
             }
             __syncthreads();
             cx<T> v2[P];
@@ -1683,12 +1687,15 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
         if (blk * lines < G.ny / 2) {
             cx<T>(*const vv[1])[P] = {&v};
             cx<T>* const rr[1] = {lds2};
-            // the inputs in row-pair tiles or per field ([kx][y])
+            // the inputs in row-pair tiles or per field ([kx][y]); one line
+            // per workgroup, whose row pair exists inside this branch
+            static_assert(NT * P / LC == 1, "one row pair per workgroup");
+            constexpr bool live = true;
             const SlotElems se = row_slots<LC>(j, G.ny, (G.nkx_keep + 1) & ~1, y0, a.pairs != 0);
             RowPairRegs<T> r;
-            gather_two_rows<LC>(r, j, a.mixed[0] + moff, se, active);
+            gather_two_rows<LC>(r, j, a.mixed[0] + moff, se, live);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
-            gather_two_rows<LC>(r, j, a.mixed[1] + moff, se, active);
+            gather_two_rows<LC>(r, j, a.mixed[1] + moff, se, live);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1, true>(vv, rr, j, tw);  // psi_x
@@ -1696,7 +1703,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             for (int s = 0; s < P; ++s) X[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
-            gather_two_rows<LC>(r, j, a.mixed[2] + moff, se, active);
+            gather_two_rows<LC>(r, j, a.mixed[2] + moff, se, live);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1, true>(vv, rr, j, tw);  // psi_y
@@ -1705,7 +1712,7 @@ __global__ __launch_bounds__(NT, MW) void k_row(RowArgs<T> a) {
             for (int s = 0; s < P; ++s) Jp[s] = v[s];
             __builtin_amdgcn_sched_barrier(0);
             stage_two_rows_regs<LC, true, false>(lds, j, r);
-            gather_two_rows<LC>(r, j, a.mixed[3] + moff, se, active);
+            gather_two_rows<LC>(r, j, a.mixed[3] + moff, se, live);
             __syncthreads();
             lds_read_kept<LC, false>(v, lds, j);
             ws_a<+1, 1, true>(vv, rr, j, tw);  // omega_x
