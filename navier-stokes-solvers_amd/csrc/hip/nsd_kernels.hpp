@@ -660,8 +660,13 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                                                          : 1;
     const int kxi = xcd_group_remap<XG>(blockIdx.x, gridDim.x) * lines + m;
     const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
-    const bool active = kxi < ncols;
-    const bool has_data = active && kxi < G.nkx_keep;
+    // The stage launches cover exactly the kept columns (launch_col_fused);
+    // with one column per workgroup every workgroup has one, and the
+    // compiler drops the zero fills and masks of an idle column.
+    constexpr bool ALL_KEPT =
+        (MODE == COL_STAGE || MODE == COL_PRIME) && LC > 0 && NT * P / LC == 1;
+    const bool active = ALL_KEPT || kxi < ncols;
+    const bool has_data = ALL_KEPT || (active && kxi < G.nkx_keep);
     const T kx = T(kxi) * T(2.0 * kPi / G.Lx);
     const size_t mbase = (size_t)kxi * G.ny;
     const cx<T> zero = mk(T(0), T(0));
