@@ -597,9 +597,9 @@ constexpr int region_elems() {
 }
 
 // store (first, second) at pair element e of pair buffer B (one 16 bytes)
-__device__ __forceinline__ void store_pair(cx<float>* B, unsigned e, cx<float> first,
+__device__ __forceinline__ void store_pair(cx<float>* B, size_t e, cx<float> first,
                                            cx<float> second) {
-    *reinterpret_cast<float4*>(at32(B, 2 * e)) = make_float4(first.x, first.y, second.x, second.y);
+    *reinterpret_cast<float4*>(B + 2 * e) = make_float4(first.x, first.y, second.x, second.y);
 }
 
 template <typename T>
@@ -678,11 +678,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     const bool active = ALL_KEPT || kxi < ncols;
     const bool has_data = ALL_KEPT || (active && kxi < G.nkx_keep);
     const T kx = T(kxi) * T(2.0 * kPi / G.Lx);
-    // Column kxi of a mixed field ([kx][y]) starts at element kxi ny, as a
-    // 32-bit element offset (a mixed field holds fewer than 2^32 / 16
-    // elements, a compact field too): loads and stores take the uniform base
-    // and one offset register, with no 64-bit address arithmetic per access.
-    const unsigned mb32 = __umul24((unsigned)kxi, (unsigned)G.ny);
+    const size_t mbase = (size_t)kxi * G.ny;
     const cx<T> zero = mk(T(0), T(0));
 
     cx<T> v[P];
@@ -721,16 +717,13 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
 #pragma unroll
             for (int s = 0; s < P; ++s) {
                 const t2 q = __builtin_nontemporal_load(
-                    reinterpret_cast<const t2*>(at32(nl, mb32 + j + s * TPL)));
+                    reinterpret_cast<const t2*>(nl + mbase + j + s * TPL));
                 v[s] = has_data ? mk(q.x, q.y) : zero;
             }
         } else {
 #pragma unroll
-            for (int s = 0; s < P; ++s) v[s] = has_data ? *at32(nl, mb32 + j + s * TPL) : zero;
+            for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
         }
-        // the accumulator only from stage 1 on, where the update reads it
-        // (a uniform branch around whole loads)
-        const bool with_acc = a.stage > 0;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             const int kind = slot_kind(s, LC, Band<LC>::kym);
@@ -738,11 +731,9 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             if (kind == 0 || !has_data) continue;
             const bool pos = kind == 1 || (kind == 3 && jy <= kym);
             if (kind == 3 && !pos && jy < Ll - kym) continue;
-            const unsigned ci = __umul24((unsigned)kxi, (unsigned)ncy) +
-                                 (unsigned)(pos ? jy : jy - (Ll - ncy));
-            pw0[s] = *at32(state, ci);
-            pacc[s] = zero;
-            if (with_acc) pacc[s] = *at32(acc, ci);
+            const size_t ci = (size_t)kxi * ncy + (pos ? jy : jy - (Ll - ncy));
+            pw0[s] = state[ci];
+            pacc[s] = a.stage > 0 ? acc[ci] : zero;
         }
         if constexpr (WS) {
             cx<T>(*const vv[1])[P] = {&v};
@@ -762,7 +753,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                 v[s] = has_data ? *at32(nl, jblock_elem(kxi, G.ny, j + s * TPL)) : zero;
         } else {
 #pragma unroll
-            for (int s = 0; s < P; ++s) v[s] = has_data ? *at32(nl, mb32 + j + s * TPL) : zero;
+            for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
         }
         fft_any<-1, LC>(v, lds, j, g, a.tw);
     }
@@ -816,59 +807,58 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
         if (kind == 0 || !has_data) continue;
         const bool pos = kind == 1 || (kind == 3 && jy <= kym);
         if (kind == 3 && !pos && jy < Ll - kym) continue;
-        const unsigned ci = __umul24((unsigned)kxi, (unsigned)ncy) +
-                                 (unsigned)(pos ? jy : jy - (Ll - ncy));
+        const size_t ci = (size_t)kxi * ncy + (pos ? jy : jy - (Ll - ncy));
         const T ky = T(pos ? jy : jy - Ll) * T(2.0 * kPi / G.Ly);
         if constexpr (MODE == COL_PRIME) {
-            w[s] = *at32(state, ci);
+            w[s] = state[ci];
             continue;
         }
         if constexpr (MODE == COL_INV) {
-            w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : *at32(a.in0 + mb * a.bc, ci);
+            w[s] = a.in_ref ? ld_ref<T>(a.in0_ref, (size_t)jy * G.nkx + kxi) : a.in0[mb * a.bc + ci];
             continue;
         }
         if constexpr (ERR && MODE == COL_STAGE) {
             if (redo) {  // the state of the last accepted step comes back
-                w[s] = *at32(prev, ci);
-                *at32(state, ci) = w[s];
+                w[s] = prev[ci];
+                state[ci] = w[s];
                 continue;
             }
         }
         // COL_STAGE: N = -norm * FFT(J) (spectral_core.c:106-111)
         cx<T> N = (-norm) * v[s];
-        if (src) N = N + *at32(src, ci);  // learned subgrid tendency, frozen over the step
-        const cx<T> w0 = PF ? pw0[s] : *at32(state, ci);
+        if (src) N = N + src[ci];  // learned subgrid tendency, frozen over the step
+        const cx<T> w0 = PF ? pw0[s] : state[ci];
         if constexpr (ERR) {
-            if (a.stage == 0) *at32(prev, ci) = w0;
+            if (a.stage == 0) prev[ci] = w0;
         }
         const T k2 = kx * kx + ky * ky;
         cx<T> wn;
         if constexpr (SCHEME == 0) {
             // classic RK4 (spectral_time_integration.c:50-100) with
             // hyperviscosity_apply (hyperviscosity.c:104-150) and drag
-            const cx<T> ws = (a.stage == 0) ? w0 : *at32(stagebuf, ci);
+            const cx<T> ws = (a.stage == 0) ? w0 : stagebuf[ci];
             const T lin = -(k2 * T(ph.inv_Re) + T(ph.drag) + hv_weight_t<T>(ph, k2));
             cx<T> ks = N + lin * ws;
             const T sw = sfd_weight_t<T>(ph, k2);
-            if (sw > T(0)) ks = ks - sw * (ws - *at32(sfd, ci));
+            if (sw > T(0)) ks = ks - sw * (ws - sfd[ci]);
             if (a.stage == 0) {
-                *at32(acc, ci) = w0 + (h / T(6)) * ks;
+                acc[ci] = w0 + (h / T(6)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
             } else if (a.stage == 1) {
-                *at32(acc, ci) = (PF ? pacc[s] : *at32(acc, ci)) + (h / T(3)) * ks;
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * ks;
                 wn = w0 + (T(0.5) * h) * ks;
-                if constexpr (ERR) *at32(ymid, ci) = w0 + h * ks;  // explicit midpoint solution
+                if constexpr (ERR) ymid[ci] = w0 + h * ks;  // explicit midpoint solution
             } else if (a.stage == 2) {
-                *at32(acc, ci) = (PF ? pacc[s] : *at32(acc, ci)) + (h / T(3)) * ks;
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * ks;
                 wn = w0 + h * ks;
             } else {
-                wn = (PF ? pacc[s] : *at32(acc, ci)) + (h / T(6)) * ks;
+                wn = (PF ? pacc[s] : acc[ci]) + (h / T(6)) * ks;
                 if (ph.sfd_on && ph.sfd_tau > 0.0) {  // hyperviscosity.c:152-178
                     const T al = T(sfd_alpha);
-                    *at32(sfd, ci) = (T(1) - al) * *at32(sfd, ci) + al * wn;
+                    sfd[ci] = (T(1) - al) * sfd[ci] + al * wn;
                 }
             }
-            if (a.stage < 3) *at32(stagebuf, ci) = wn;
+            if (a.stage < 3) stagebuf[ci] = wn;
         } else {
             // integrating factor, L = -(k^2/Re + w_hv + mu).  The factors are
             // carried as E - 1 (expm1): near k = 0 the per-step decay L h is
@@ -878,23 +868,23 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             const T e2 = expm1_t(L * (T(0.5) * h));  // E2 - 1, E2 = exp(L h / 2)
             const T e1 = e2 * (T(2) + e2);          // E - 1,  E = E2^2
             if (a.stage == 0) {
-                *at32(acc, ci) = grow(w0, e1) + (h / T(6)) * grow(N, e1);
+                acc[ci] = grow(w0, e1) + (h / T(6)) * grow(N, e1);
                 wn = grow(w0 + (T(0.5) * h) * N, e2);
             } else if (a.stage == 1) {
-                *at32(acc, ci) = (PF ? pacc[s] : *at32(acc, ci)) + (h / T(3)) * grow(N, e2);
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e2) + (T(0.5) * h) * N;
                 // Lawson midpoint solution E w0 + h E2 N(b)
-                if constexpr (ERR) *at32(ymid, ci) = grow(w0, e1) + h * grow(N, e2);
+                if constexpr (ERR) ymid[ci] = grow(w0, e1) + h * grow(N, e2);
             } else if (a.stage == 2) {
-                *at32(acc, ci) = (PF ? pacc[s] : *at32(acc, ci)) + (h / T(3)) * grow(N, e2);
+                acc[ci] = (PF ? pacc[s] : acc[ci]) + (h / T(3)) * grow(N, e2);
                 wn = grow(w0, e1) + h * grow(N, e2);
             } else {
-                wn = (PF ? pacc[s] : *at32(acc, ci)) + (h / T(6)) * N;
+                wn = (PF ? pacc[s] : acc[ci]) + (h / T(6)) * N;
             }
         }
         if (a.stage == 3) {
             if constexpr (ERR) {  // embedded estimate, before the forcing kick
-                const cx<T> e = wn - *at32(ymid, ci);
+                const cx<T> e = wn - ymid[ci];
                 const double wt = kxi == 0 ? 1.0 : 2.0;
                 esum += wt * ((double)e.x * (double)e.x + (double)e.y * (double)e.y);
                 ssum += wt * ((double)wn.x * (double)wn.x + (double)wn.y * (double)wn.y);
@@ -904,7 +894,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                 if (b >= -ph.fby && b <= ph.fby)
                     wn = wn + sqh * fbox[kxi * (2 * ph.fby + 1) + (b + ph.fby)];
             }
-            *at32(state, ci) = wn;
+            state[ci] = wn;
         }
         w[s] = wn;
     }
@@ -969,7 +959,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             if (has_data) {
                 cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
-                for (int s = 0; s < P; ++s) *at32(out, mb32 + j + s * TPL) = v[s];
+                for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
     } else if constexpr (paired_lines<T, LC, NT, PR>()) {
@@ -996,7 +986,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                     if (a.pairs) {  // (omega_x, omega_y)
                         cx<T>* pb = a.mixed[2] + 2 * mixed_off;
 #pragma unroll
-                        for (int s = 0; s < P; ++s) store_pair(pb, mb32 + j + s * TPL, v2[s], v[s]);
+                        for (int s = 0; s < P; ++s) store_pair(pb, mbase + j + s * TPL, v2[s], v[s]);
                     }
                 }
                 if (!(pair_layout<T, LC, NT, PR>() && a.pairs)) {
@@ -1004,8 +994,8 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                     cx<T>* o2 = a.mixed[2] + mixed_off;
 #pragma unroll
                     for (int s = 0; s < P; ++s) {
-                        *at32(o1, mb32 + j + s * TPL) = v[s];
-                        *at32(o2, mb32 + j + s * TPL) = v2[s];
+                        o1[mbase + j + s * TPL] = v[s];
+                        o2[mbase + j + s * TPL] = v2[s];
                     }
                 }
             }
@@ -1023,7 +1013,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                     if (a.pairs) {  // (psi_x, psi_y)
                         cx<T>* pb = a.mixed[0] + 2 * mixed_off;
 #pragma unroll
-                        for (int s = 0; s < P; ++s) store_pair(pb, mb32 + j + s * TPL, v2[s], v[s]);
+                        for (int s = 0; s < P; ++s) store_pair(pb, mbase + j + s * TPL, v2[s], v[s]);
                     }
                 }
                 if (!(pair_layout<T, LC, NT, PR>() && a.pairs)) {
@@ -1031,8 +1021,8 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                     cx<T>* o2 = a.mixed[0] + mixed_off;
 #pragma unroll
                     for (int s = 0; s < P; ++s) {
-                        *at32(o1, mb32 + j + s * TPL) = v[s];
-                        *at32(o2, mb32 + j + s * TPL) = v2[s];
+                        o1[mbase + j + s * TPL] = v[s];
+                        o2[mbase + j + s * TPL] = v2[s];
                     }
                 }
             }
@@ -1060,8 +1050,8 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                 cx<T>* o2 = a.mixed[f - 1] + mixed_off;
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
-                    *at32(o1, mb32 + j + s * TPL) = v[s];
-                    *at32(o2, mb32 + j + s * TPL) = v2[s];
+                    o1[mbase + j + s * TPL] = v[s];
+                    o2[mbase + j + s * TPL] = v2[s];
                 }
             }
         }
@@ -1126,7 +1116,7 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             if (has_data) {
                 cx<T>* out = a.mixed[f] + mixed_off;
 #pragma unroll
-                for (int s = 0; s < P; ++s) *at32(out, mb32 + j + s * TPL) = v[s];
+                for (int s = 0; s < P; ++s) out[mbase + j + s * TPL] = v[s];
             }
         }
     }
