@@ -672,9 +672,12 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
     const int ncols = (MODE == COL_FWD) ? G.nkx : G.nkx_keep;
     // The stage launches cover exactly the kept columns (launch_col_fused);
     // with one column per workgroup every workgroup has one, and the
-    // compiler drops the zero fills and masks of an idle column.
-    constexpr bool ALL_KEPT =
-        (MODE == COL_STAGE || MODE == COL_PRIME) && LC > 0 && NT * P / LC == 1;
+    // compiler drops the zero fills and masks of an idle column.  Taken for
+    // the prime pass and the integrating-factor stage up to 4096 points: in
+    // the classic scheme with the error controller, and at 8192, the freed
+    // schedule costs the compiler registers it then spills.
+    constexpr bool ALL_KEPT = (MODE == COL_PRIME || (MODE == COL_STAGE && SCHEME == 1)) &&
+                              LC > 0 && LC <= 4096 && NT * P / LC == 1;
     const bool active = ALL_KEPT || kxi < ncols;
     const bool has_data = ALL_KEPT || (active && kxi < G.nkx_keep);
     const T kx = T(kxi) * T(2.0 * kPi / G.Lx);
