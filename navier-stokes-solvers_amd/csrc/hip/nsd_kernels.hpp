@@ -727,6 +727,9 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
 #pragma unroll
             for (int s = 0; s < P; ++s) v[s] = has_data ? nl[mbase + j + s * TPL] : zero;
         }
+        // the accumulator only from stage 1 on, where the update reads it (a
+        // uniform branch around whole loads, which a select would split)
+        const bool with_acc = a.stage > 0;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             const int kind = slot_kind(s, LC, Band<LC>::kym);
@@ -736,7 +739,8 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
             if (kind == 3 && !pos && jy < Ll - kym) continue;
             const size_t ci = (size_t)kxi * ncy + (pos ? jy : jy - (Ll - ncy));
             pw0[s] = state[ci];
-            pacc[s] = a.stage > 0 ? acc[ci] : zero;
+            pacc[s] = zero;
+            if (with_acc) pacc[s] = acc[ci];
         }
         if constexpr (WS) {
             cx<T>(*const vv[1])[P] = {&v};
