@@ -985,9 +985,20 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                 v[s] = mul_i(w[s], ky);
                 v2[s] = mul_i(w[s], kx);
             }
-            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // omega_y, omega_x
-            TwRegs<LC, T> tw;
-            load_tw_regs<LC, 0>(tw, j, a.tw);
+            // FP32 4096 (pair layout): wave-split transforms, standard
+            // pattern in and out (ws_c), transposing back in the same regions
+            constexpr bool WSC = pair_layout<T, LC, NT, PR>();
+            std::conditional_t<WSC, TwWs<T>, TwRegs<LC, T>> tw;
+            cx<T>(*const vv[2])[P] = {&v, &v2};
+            cx<T>* const rr[2] = {lds, lds2};
+            if constexpr (WSC) {
+                load_tw_ws(tw, j, a.tw);
+                __syncthreads();  // the forward transform's last reads are done
+                ws_c<+1, 2>(vv, rr, rr, j, tw);  // omega_y, omega_x
+            } else {
+                line_fft2_c<+1, LC>(v, v2, lds, lds2, j, a.tw);  // omega_y, omega_x
+                load_tw_regs<LC, 0>(tw, j, a.tw);
+            }
             if (has_data) {
                 if constexpr (pair_layout<T, LC, NT, PR>()) {
                     if (a.pairs) {  // (omega_x, omega_y)
@@ -1014,7 +1025,12 @@ __global__ __launch_bounds__(NT, MW) void k_col(ColArgs<T> a) {
                 v[s] = mul_i(psi, ky);
                 v2[s] = mul_i(psi, kx);
             }
-            line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_y, psi_x
+            if constexpr (WSC) {
+                __syncthreads();  // others may still read the transposed first pair
+                ws_c<+1, 2>(vv, rr, rr, j, tw);  // psi_y, psi_x
+            } else {
+                line_fft2_c<+1, LC>(v, v2, lds, lds2, j, tw);  // psi_y, psi_x
+            }
             if (has_data) {
                 if constexpr (pair_layout<T, LC, NT, PR>()) {
                     if (a.pairs) {  // (psi_x, psi_y)
