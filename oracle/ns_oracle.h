@@ -27,12 +27,15 @@
  *                    oracle/_ref/ and against committed golden vectors
  *   energy/enstrophy Taylor-Green closed forms E = 0.25 e^{-4 nu t}, Z = 0.5 e^{-4 nu t}
  *   spectrum         shell sums vs numpy binning of the same modes
+ *   RK4, generic J   the whole step on a J != 0 field: one-step error against 64
+ *                    finer steps falls as dt^5 (ratio 32 per halving; a stage
+ *                    built from the wrong field gives 8); inviscid, energy and
+ *                    enstrophy change by the truncation error only
  *
- * Parity unpinned: the complete classic RK4 step on a generic field (J != 0)
- * as one composite.  No compiled reference step exists to compare it with
- * (the reference's spectral path needs FFTW, absent here); it is pinned only
- * through its parts above and through the test that the minimal step equals
- * the reference's own 40-transform call sequence.
+ * No compiled reference step exists to compare the whole step with (the
+ * reference's spectral path needs FFTW, absent here); the properties above and
+ * the test that the minimal step equals the reference's own 40-transform call
+ * sequence pin it instead.
  */
 #ifndef NS_ORACLE_H
 #define NS_ORACLE_H

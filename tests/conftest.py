@@ -46,3 +46,23 @@ def rel_l2(a, b):
     b = np.asarray(b)
     den = np.linalg.norm(b.ravel())
     return np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0)
+
+
+def rk4_one_step_errors(step, w0, dts, substeps=64):
+    """One-step error of `step(w, dt, m)` (m steps of dt/m from w) against the
+    same system stepped with `substeps` steps, for each dt."""
+    errs = []
+    for dt in dts:
+        one, fine = step(w0, dt, 1), step(w0, dt, substeps)
+        errs.append(np.linalg.norm(one - fine) / np.linalg.norm(fine))
+    return np.array(errs)
+
+
+def quadratic_invariants(w, k2):
+    """Parseval sums proportional to E and Z of a half spectrum (kx = 0 once)."""
+    wt = np.ones_like(k2)
+    wt[:, 1:] = 2.0
+    a2 = np.abs(w) ** 2
+    return (np.sum(wt * a2 * np.where(k2 > 0, 1.0 / np.maximum(k2, 1e-300), 0.0)),
+            np.sum(wt * a2))
+

@@ -6,7 +6,7 @@ assertions restated, and closed forms.  Tolerances: FP64 relative L2 1e-11
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import quadratic_invariants, rel_l2, rk4_one_step_errors
 
 pytestmark = pytest.mark.gpu
 TWO_PI = 2 * np.pi
@@ -178,6 +178,51 @@ def test_generic_field_classic_rk4_adaptive_vs_oracle(ns, oc, nx, ny):
     assert b.sync_host() == 0
     assert rel_l2(b.spec("omega_hat"), o.spec("omega_hat")) <= 1e-11
     assert b.d.t == pytest.approx(o.d.t, rel=1e-12)
+
+
+def test_generic_field_classic_rk4_is_fourth_order_and_conservative(ns, oc):
+    """The device's classic RK4 step on a generic field (J != 0), through
+    ns_spectral_advance_rk4, held to what the mathematics fixes for the
+    composite (spectral_time_integration.c:34-106): its one-step error
+    against the same device system stepped 64 times finer falls as dt^5
+    (ratio 32 per halving of dt; a stage built from the wrong field gives 8),
+    and inviscid, one step changes energy and enstrophy by the truncation
+    error only. The oracle's step meets the same bounds on the same field."""
+    n = 32
+    s = solver(ns, n, Re=1e300)  # 1/Re = 1e-300: the inviscid system
+    s.init(taylor_green=False)
+    s.random_phase(5, k_peak=3.0)
+    o = oracle(oc, n, Re=1e300)
+    o.init_random_phase(5, k_peak=3.0)
+    w0 = s.spec("omega_hat").copy()
+    assert rel_l2(w0, o.spec("omega_hat")) < 1e-14
+    k2 = o.k2.reshape(w0.shape)
+
+    def device_steps(w, dt, m):
+        s.spec("omega_hat")[:] = w
+        for _ in range(m):
+            s.d.dt = dt / m
+            assert s.advance_rk4() == 0
+        return s.spec("omega_hat").copy()
+
+    dts = (0.02, 0.01, 0.005)
+    errs = rk4_one_step_errors(device_steps, w0, dts)
+    ratios = errs[:-1] / errs[1:]
+    print("one-step errors", errs, "ratios", ratios, flush=True)
+    assert np.all((ratios > 28.0) & (ratios < 36.0)), ratios
+    assert errs[-1] < 1e-8
+    E0, Z0 = quadratic_invariants(w0, k2)
+    for dt, bound in ((0.01, 1e-8), (0.005, 3e-10)):
+        w1 = device_steps(w0, dt, 1)
+        E1, Z1 = quadratic_invariants(w1, k2)
+        print(f"dt {dt}: dE {abs(E1 / E0 - 1):.2e} dZ {abs(Z1 / Z0 - 1):.2e}", flush=True)
+        assert abs(E1 / E0 - 1) < bound and abs(Z1 / Z0 - 1) < 10 * bound
+        o.spec("omega_hat")[:] = w0
+        o.compute_derivatives()
+        o.d.dt = dt
+        o.advance_rk4()
+        assert rel_l2(w1, o.spec("omega_hat")) < 1e-12
+    s.close()
 
 
 def test_hyperviscosity_and_sfd_classic_vs_oracle(ns, oc):

@@ -4,7 +4,7 @@ test_conservation.c), so that a dropped term, wrong sign or index fails."""
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import quadratic_invariants, rel_l2, rk4_one_step_errors
 
 TWO_PI = 2 * np.pi
 
@@ -450,3 +450,60 @@ def test_if_rk4_energy_enstrophy_budget_closes(oc):
     for X, dX in ((E, dE), (Z, dZ)):
         integral = dt / 3.0 * np.sum(simpson * dX)
         assert abs((X[-1] - X[0]) - integral) <= 1e-9 * abs(integral)
+
+
+def test_classic_rk4_generic_field_is_fourth_order_and_conservative(oc):
+    """The classic RK4 step on a generic field (J != 0) as one composite
+    (spectral_time_integration.c:34-106), pinned by what the mathematics
+    fixes rather than by a second copy of the formula:
+    - its one-step error against the same system stepped 64 times finer falls
+      as dt^5, so each halving of dt divides it by 32. A step whose stages take
+      the wrong field (stage 3 built from k1 instead of k2) is second order and
+      gives 8, planted below in a composition of the oracle's own nonlinear
+      term; weights other than (1, 2, 2, 1)/6 drop the order the same way;
+    - inviscid, the dealiased Jacobian conserves energy and enstrophy, so one
+      step changes them by the step's truncation error only. A dropped or
+      transposed product in J changes them at first order in dt."""
+    n = 32
+    o = oc.Oracle(n, n, Re=1e300)  # 1/Re = 1e-300: the inviscid system
+    o.init_grids()
+    o.init_random_phase(5, k_peak=3.0, energy=1.0)
+    w0 = o.spec("omega_hat").copy()
+    k2 = o.k2.reshape(w0.shape)
+
+    def oracle_steps(w, dt, m):
+        o.spec("omega_hat")[:] = w
+        o.compute_derivatives()
+        for _ in range(m):
+            o.d.dt = dt / m
+            o.advance_rk4()
+        return o.spec("omega_hat").copy()
+
+    dts = (0.02, 0.01, 0.005)
+    errs = rk4_one_step_errors(oracle_steps, w0, dts)
+    ratios = errs[:-1] / errs[1:]
+    assert np.all((ratios > 28.0) & (ratios < 36.0)), ratios
+    assert errs[-1] < 1e-8
+    E0, Z0 = quadratic_invariants(w0, k2)
+    for dt, bound in ((0.01, 1e-8), (0.005, 3e-10)):
+        E1, Z1 = quadratic_invariants(oracle_steps(w0, dt, 1), k2)
+        assert abs(E1 / E0 - 1) < bound and abs(Z1 / Z0 - 1) < 10 * bound
+
+    # the same measure on a planted mistake: stage 3 from k1 instead of k2
+    def f(w):
+        o.spec("omega_hat")[:] = w
+        o.compute_derivatives()
+        o.nonlinear()
+        return o.spec("nonlinear_hat").copy()
+
+    def planted_steps(w, dt, m):
+        if m > 1:
+            return oracle_steps(w, dt, m)
+        k1 = f(w)
+        k2_ = f(w + dt / 2 * k1)
+        k3 = f(w + dt / 2 * k1)
+        k4 = f(w + dt * k3)
+        return w + dt / 6 * (k1 + 2 * k2_ + 2 * k3 + k4)
+
+    bad = rk4_one_step_errors(planted_steps, w0, dts)
+    assert np.all(bad[:-1] / bad[1:] < 10.0)
