@@ -7,7 +7,8 @@ grid) and 2048^2 (the closure's fine grid).  The GPU starts from the oracle's
 random-phase field (uploaded), so both sides step identical inputs with
 identical forcing draws:
   FP64: two steps, relative L2 of omega_hat <= 1e-11, t to 1e-13;
-  FP32 stages: one step, relative L2 <= 1e-5 against the FP64 oracle.
+  FP32 stages: one step, relative L2 <= 1e-5 against the FP64 oracle;
+  FP64, forcing off: the one-step error falls as dt^5 (fourth order).
 The step being checked is the reference's RK4 step
 (src/spectral/spectral_time_integration.c:34-106) with the integrating
 factor, forcing and drag of the workload; the oracle restates it in FP64 C.
@@ -15,7 +16,7 @@ factor, forcing and drag of the workload; the oracle restates it in FP64 C.
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, rk4_one_step_errors
 
 pytestmark = pytest.mark.gpu
 TWO_PI = 2 * np.pi
@@ -120,4 +121,41 @@ def test_forced_workload_fp32_vs_oracle(ns, oc, n):
           f"|k| <= 32 rel L2 {err_low:.2e}; dt {s.d.dt:.8e} vs {dt_ref:.8e}", flush=True)
     assert s.d.dt == pytest.approx(dt_ref, rel=1e-6)
     assert err <= 1e-5 and err_low <= 1e-5
+    s.close()
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_integrating_factor_step_is_fourth_order(ns, n):
+    """The workload's integrating-factor RK4 step (Lawson form of
+    spectral_time_integration.c:34-106 with the viscous, drag and del^8 terms
+    as the factor), forcing off, through the 2048- and 4096-point FP64 stage
+    kernels, held to what the mathematics fixes without the oracle: its
+    one-step error against the same device system stepped 64 times finer
+    falls as dt^5, 32 per halving of dt. A stage that takes the wrong field
+    or the wrong factor is of lower order and misses the ratio: E for E^(1/2)
+    on the third stage's tendency, planted in a numpy Lawson composition of
+    the oracle's nonlinear term on this workload at 256^2, gives 8.0, where
+    the oracle's own step gives 32.0."""
+    s = ns.Spectral(n, n, TWO_PI, TWO_PI, RE)
+    s.init(taylor_green=False)
+    s.hv_set(**hv(n))
+    s.set_precision(False)
+    s.set_scheme(True)
+    s.set_drag(DRAG)
+    s.random_phase(SEED, 10.0, 1.0)
+    w0 = s.spec("omega_hat").copy()
+
+    def device_steps(w, dt, m):
+        s.spec("omega_hat")[:] = w
+        s.d.t = 0.0
+        s.d.dt = dt / m
+        s.sync_device()
+        assert s.advance_n(m, False) == 0
+        return download(ns, s)
+
+    errs = rk4_one_step_errors(device_steps, w0, (1e-3, 5e-4, 2.5e-4))
+    ratios = errs[:-1] / errs[1:]
+    print(f"{n}^2 one-step errors {errs}, ratios {ratios}", flush=True)
+    assert np.all((ratios > 28.0) & (ratios < 36.0)), ratios
+    assert errs[-1] < 1e-10
     s.close()
